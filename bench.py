@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the sparse x dense execute path on one MI355X.
+
+Workload (the headline configuration): FP64 SpMV y = A x on a random
+power-law matrix, n = 4,194,304 rows and columns, row lengths
+round(lognormal(2.5, 1.0)) clipped to [1, 20000], uniform distinct sorted
+columns, N(0,1) values, x uniform[-1, 1] (seeded synthetic data).
+
+A step is one SpMV over the whole matrix with x and y resident in HBM,
+issued through perflibs_amd_spmv_exec_async_d on the kernel variant the
+library's optimize() selected.  `value` is algorithmic GB/s of the timed steps:
+    bytes = nnz*(8 + 4) + (n + 1)*4 + n*8 (x) + n*8 (y written, beta = 0)
+(values + column indices + row pointers + x + y, each counted once).
+
+Also reported: every SpMV variant's GB/s on the same matrix (the exhaustive
+judge) and the selected variant's regret, the HIP-event roofline of the
+dominant kernel, and the reference's CPU path (this project's OpenMP port of
+it) timed on the host.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "perf-libs-sparse_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+N_ROWS = 4_194_304
+SEED_A, SEED_X = 0x5EED0002, 0x5EED1002
+METRIC = "SpMV/SpMM GB/s (% of HBM peak) + learned-vs-exhaustive kernel regret"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n: int, nnz: int) -> int:
+    return nnz * (8 + 4) + (n + 1) * 4 + n * 8 + n * 8
+
+
+def time_gpu(A, xd, yd, warmup: int, steps: int):
+    import torch
+
+    for _ in range(warmup):
+        A.spmv_async(xd.data_ptr(), yd.data_ptr())
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        A.spmv_async(xd.data_ptr(), yd.data_ptr())
+    e1.record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    return wall_ms, e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline(n, rp, ci, v, x, y_gpu) -> dict:
+    """The reference's OpenMP CPU path (oracle/cpu.c, oracle/scs.c port):
+    nnz-balanced first-touch CSR (K3), vanilla CSR (K1) and the generic
+    SELL-C-sigma kernel with the audition's candidate rule (first C from the
+    FP64 registry 16, 14, ..., 2 and sigma = C*{0, 200, 2000} whose padding
+    efficiency reaches 0.96); the fastest is the CPU's pick, as the
+    reference's optimize() would choose."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    L = O.lib()
+    threads = O.omp_threads()
+    p = lambda a: a.ctypes.data
+    nnz = int(rp[-1])
+    reps, warm = 10, 3
+    y = np.zeros(n)
+
+    def timed(fn):
+        for _ in range(warm):
+            fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    results = {}
+    rows, inds = O.csr_decomp(n, rp, threads, 1)
+    results["csr_first_touch"] = timed(lambda: L.orc_cpu_csr_first_touch(
+        n, p(rp), p(ci), p(v), p(x), 1.0, 0.0, p(y), threads, p(rows), p(inds)))
+    results["csr_vanilla"] = timed(lambda: L.orc_cpu_csr_vanilla(n, p(rp), p(ci), p(v), p(x), 1.0, 0.0, p(y)))
+    pick = None
+    for Cc in (16, 14, 12, 10, 8, 6, 4, 2):
+        for sm in (0, 200, 2000):
+            if O.scs_beta(n, n, rp, ci, Cc, Cc * sm) >= 0.96:
+                pick = (Cc, Cc * sm)
+                break
+        if pick:
+            break
+    if pick:
+        h = O.ScsHandle(n, n, rp, ci, v, *pick)
+        results[f"sell_c{pick[0]}_s{pick[1]}"] = timed(lambda: h.spmv(x, y))
+        h.close()
+    best = min(results, key=results.get)
+    t = results[best]
+    # the checker: the GPU result against the CPU port's y of the best kernel
+    y[:] = 0
+    if best == "csr_vanilla":
+        L.orc_cpu_csr_vanilla(n, p(rp), p(ci), p(v), p(x), 1.0, 0.0, p(y))
+    else:
+        L.orc_cpu_csr_first_touch(n, p(rp), p(ci), p(v), p(x), 1.0, 0.0, p(y), threads, p(rows), p(inds))
+    diff = float(np.linalg.norm(y - y_gpu) / np.linalg.norm(y))
+    gbs = algorithmic_bytes(n, nnz) / t / 1e9
+    return {
+        "value": round(gbs, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": f"full matrix (nnz={nnz}), {warm} warm-up + median of {reps} SpMV per kernel, "
+                  f"best of {sorted(results)}",
+        "kernel": best, "ms": round(t * 1e3, 3),
+        "all_ms": {k: round(s * 1e3, 3) for k, s in results.items()},
+        "gpu_vs_cpu_rel_diff": diff,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--variant", default="auto", help="force a variant name, or 'auto' (optimize selects)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the per-variant exhaustive sweep")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--rows", type=int, default=N_ROWS)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU is not part of this build",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from plsparse import workloads as W
+
+    n = args.rows
+    t0 = time.perf_counter()
+    rp, ci, v = W.powerlaw(n, SEED_A)
+    x = W.uniform(n, SEED_X)
+    nnz = int(rp[-1])
+    log(f"matrix n={n} nnz={nnz} generated in {time.perf_counter() - t0:.1f}s")
+    nbytes = algorithmic_bytes(n, nnz)
+
+    stream = torch.cuda.current_stream()
+    P.set_stream(stream.cuda_stream)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    A = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+    A.hint(P.HINT_SPMV_INVOCATIONS, P.INVOCATIONS_MANY)
+    if args.variant != "auto":
+        A.force_variant(P.VARIANTS.index(args.variant))
+    t0 = time.perf_counter()
+    A.optimize()
+    chosen = P.VARIANTS[A.variant()]
+    log(f"optimize() selected {chosen} in {time.perf_counter() - t0:.1f}s")
+
+    wall_ms, kern_ms = time_gpu(A, xd, yd, args.warmup, args.steps)
+    y_gpu = yd.cpu().numpy()
+    value = nbytes / (wall_ms * 1e-3) / 1e9
+    achieved = nbytes / (kern_ms * 1e-3) / 1e9
+    log(f"{chosen}: {wall_ms:.3f} ms/step wall, {kern_ms:.3f} ms kernel, {achieved:.1f} GB/s")
+    A.destroy()
+
+    sweep = {}
+    if not args.no_sweep:
+        for vi, name in enumerate(P.VARIANTS):
+            if name.startswith("bsr"):
+                continue  # random columns: 4x4/8x8 blocks would hold ~1 value each
+            B = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+            B.force_variant(vi)
+            B.optimize()
+            _, kms = time_gpu(B, xd, yd, 3, 20)
+            sweep[name] = kms
+            log(f"  sweep {name}: {kms:.3f} ms  {nbytes / kms / 1e6:.1f} GB/s")
+            B.destroy()
+    best = min(sweep, key=sweep.get) if sweep else chosen
+    regret = (sweep[chosen] / sweep[best] - 1.0) if chosen in sweep else None
+
+    cpu = None if args.no_cpu else cpu_baseline(n, rp, ci, v, x, y_gpu)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "spmv_fp64_powerlaw_4m", "rows": n, "cols": n, "nnz": nnz,
+            "row_length": "round(lognormal(2.5, 1.0)) clipped to [1, 20000]",
+            "alpha": 1.0, "beta": 0.0, "seed": SEED_A, "variant": chosen,
+            "hbm_fraction": round(value / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": chosen, "kernel_ms": round(kern_ms, 4), "algorithmic_bytes": nbytes,
+        },
+        "cpu_baseline": cpu,
+        "variants_gbs": {k: round(nbytes / ms / 1e6, 1) for k, ms in sweep.items()},
+        "selector": {"picked": chosen, "exhaustive_best": best,
+                     "regret": None if regret is None else round(regret, 4)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

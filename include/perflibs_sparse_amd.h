@@ -1,0 +1,161 @@
+/*
+ * perflibs_sparse_amd.h — MI355X-specific additions to the perflibs_sparse.h
+ * interface.  Every symbol here is new (prefix perflibs_amd_); none of them
+ * exists in the reference, and nothing in perflibs_sparse.h is changed by them.
+ *
+ * Groups:
+ *   - streams and asynchronous execution on device-resident vectors
+ *     (the drop-in perflibs_spmv_exec_* is synchronous),
+ *   - forcing an SpMV kernel variant, the GPU counterpart of the reference's
+ *     internal set_C/set_sigma audition knobs (src/matrix_state.cpp:445-462),
+ *   - value storage precision (FP64 / FP32 / BF16, FP32 accumulation for the
+ *     reduced formats),
+ *   - format selection: heuristic, exhaustive autotuner, learned MLP,
+ *   - inspection of the device layout (tests) and batched execution.
+ */
+#ifndef PERFLIBS_SPARSE_AMD_H
+#define PERFLIBS_SPARSE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "perflibs_sparse.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- SpMV kernel variants (the selector's label space) ---- */
+enum perflibs_amd_variant {
+  PERFLIBS_AMD_VARIANT_AUTO = -1,
+  PERFLIBS_AMD_CSR_VECTOR = 0,   /* sub-wave lanes per row, width from mean row length */
+  PERFLIBS_AMD_CSR_ADAPTIVE = 1, /* nnz-balanced row blocks, LDS-staged products */
+  PERFLIBS_AMD_SELL_C32_S0 = 2,  /* SELL-C-sigma, C = 32, no sorting */
+  PERFLIBS_AMD_SELL_C32_S64 = 3,
+  PERFLIBS_AMD_SELL_C32_S1024 = 4,
+  PERFLIBS_AMD_SELL_C32_S16384 = 5,
+  PERFLIBS_AMD_SELL_C64_S0 = 6,
+  PERFLIBS_AMD_SELL_C64_S64 = 7,
+  PERFLIBS_AMD_SELL_C64_S1024 = 8,
+  PERFLIBS_AMD_SELL_C64_S16384 = 9,
+  PERFLIBS_AMD_BSR_B4 = 10,      /* blocked 4x4 (converted from CSR when needed) */
+  PERFLIBS_AMD_BSR_B8 = 11,
+  PERFLIBS_AMD_NUM_VARIANTS = 12
+};
+
+/* ---- how perflibs_spmv_optimize picks a variant ---- */
+enum perflibs_amd_select_mode {
+  PERFLIBS_AMD_SELECT_LEARNED = 0,    /* MLP on matrix features (default when a model is loaded) */
+  PERFLIBS_AMD_SELECT_EXHAUSTIVE = 1, /* time every variant, keep the fastest */
+  PERFLIBS_AMD_SELECT_HEURISTIC = 2,  /* fixed rules on the same features, no network */
+  PERFLIBS_AMD_SELECT_FIXED_CSR = 3   /* always PERFLIBS_AMD_CSR_VECTOR */
+};
+
+/* ---- value storage precision ---- */
+enum perflibs_amd_precision {
+  PERFLIBS_AMD_VALUES_NATIVE = 0, /* the handle's own type (FP64 for _d, FP32 for _s) */
+  PERFLIBS_AMD_VALUES_BF16 = 2    /* _s handles only: BF16 values, FP32 x/y and accumulation */
+};
+
+#define PERFLIBS_AMD_NUM_FEATURES 12
+
+/* Stream used by every kernel launch of the library (default: the null
+ * stream).  `stream` is a hipStream_t. */
+perflibs_status_t perflibs_amd_set_stream(void *stream);
+void *perflibs_amd_get_stream(void);
+
+/* y = alpha*op(A)*x + beta*y with x and y in device memory, enqueued on the
+ * library stream and not synchronized.  Same semantics as the drop-in call. */
+perflibs_status_t perflibs_amd_spmv_exec_async_d(
+    enum perflibs_sparse_hint_value trans, double alpha, perflibs_spmat_t A,
+    const double *x_dev, double beta, double *y_dev);
+perflibs_status_t perflibs_amd_spmv_exec_async_s(
+    enum perflibs_sparse_hint_value trans, float alpha, perflibs_spmat_t A,
+    const float *x_dev, float beta, float *y_dev);
+
+/* C = alpha*A*B + beta*C with row-major device operands (ldb, ldc in
+ * elements), enqueued on the library stream and not synchronized. */
+perflibs_status_t perflibs_amd_spmm_exec_async_d(
+    double alpha, perflibs_spmat_t A, perflibs_int_t k, const double *B_dev,
+    perflibs_int_t ldb, double beta, double *C_dev, perflibs_int_t ldc);
+perflibs_status_t perflibs_amd_spmm_exec_async_s(
+    float alpha, perflibs_spmat_t A, perflibs_int_t k, const float *B_dev,
+    perflibs_int_t ldb, float beta, float *C_dev, perflibs_int_t ldc);
+
+/* Variant control and introspection. */
+perflibs_status_t perflibs_amd_set_select_mode(perflibs_spmat_t A, int mode);
+perflibs_status_t perflibs_amd_force_variant(perflibs_spmat_t A, int variant);
+perflibs_status_t perflibs_amd_get_variant(perflibs_spmat_t A, int *variant);
+perflibs_status_t perflibs_amd_set_precision(perflibs_spmat_t A, int precision);
+/* Force SELL-C-sigma with any C in [1, 64] and sigma a multiple of C (or 0). */
+perflibs_status_t perflibs_amd_force_sell(perflibs_spmat_t A, perflibs_int_t C,
+                                          perflibs_int_t sigma);
+
+/* Matrix features used by the selector (PERFLIBS_AMD_NUM_FEATURES floats),
+ * computed on the GPU from the CSR structure. */
+perflibs_status_t perflibs_amd_features(perflibs_spmat_t A, float *features);
+
+/* Time every variant of A (warmup + reps launches each, HIP events on the
+ * library stream), writing milliseconds per launch; variants that do not
+ * apply get a negative time.  Used by the exhaustive mode and to build
+ * selector training data. */
+perflibs_status_t perflibs_amd_time_variants(perflibs_spmat_t A, int warmup,
+                                             int reps, double *ms);
+
+/* Selector model: 12 -> 64 -> 64 -> PERFLIBS_AMD_NUM_VARIANTS MLP predicting
+ * log(time / best time) per variant.  Weights are a flat FP32 array
+ * (W1[12x64], b1[64], W2[64x64], b2[64], W3[64xK], b3[K]) plus the feature
+ * standardisation (mean[12], std[12]). */
+#define PERFLIBS_AMD_MLP_HIDDEN 64
+size_t perflibs_amd_mlp_num_params(void);
+perflibs_status_t perflibs_amd_set_selector_model(const float *params,
+                                                  const float *feat_mean,
+                                                  const float *feat_std);
+perflibs_status_t perflibs_amd_clear_selector_model(void);
+
+/* MLP training/inference primitives on device buffers (FP32), used by the
+ * training driver and by the gradient tests.  n = number of samples.
+ *   forward : X[n x 12] -> out[n x K]; h1, h2 are [n x 64] activations saved
+ *             for backward (may be NULL for inference).
+ *   backward: given targets T[n x K] and mask M[n x K] (1 = measured), the
+ *             masked mean squared error loss and its gradient w.r.t. the flat
+ *             params into grad (same layout as params).
+ *   adam    : one Adam step in place. */
+perflibs_status_t perflibs_amd_mlp_forward(const float *params_dev,
+                                           const float *X_dev, int n,
+                                           float *h1_dev, float *h2_dev,
+                                           float *out_dev);
+perflibs_status_t perflibs_amd_mlp_backward(const float *params_dev,
+                                            const float *X_dev, int n,
+                                            const float *h1_dev,
+                                            const float *h2_dev,
+                                            const float *out_dev,
+                                            const float *T_dev,
+                                            const float *M_dev,
+                                            float *grad_dev, float *loss_dev);
+perflibs_status_t perflibs_amd_mlp_adam(float *params_dev, const float *grad_dev,
+                                        float *m_dev, float *v_dev, int step,
+                                        float lr, float beta1, float beta2,
+                                        float eps);
+
+/* Device layout of a SELL-C-sigma handle, copied to host arrays sized from
+ * perflibs_amd_sell_info (for bit-exact layout tests).  offsets are the
+ * logical column offsets widened to int64. */
+perflibs_status_t perflibs_amd_sell_info(perflibs_spmat_t A, perflibs_int_t *C,
+                                         perflibs_int_t *sigma, int64_t *nC,
+                                         int64_t *padded,
+                                         perflibs_int_t *col_indx_bytes,
+                                         perflibs_int_t *n_full_chunks,
+                                         perflibs_int_t *ntidyup);
+perflibs_status_t perflibs_amd_sell_export(perflibs_spmat_t A, int64_t *cs,
+                                           int64_t *cl, int64_t *row_permd2in,
+                                           int64_t *col_indx_min,
+                                           int64_t *offsets, double *vals);
+
+/* Last error message of the library (thread-unsafe, for diagnostics). */
+const char *perflibs_amd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PERFLIBS_SPARSE_AMD_H */

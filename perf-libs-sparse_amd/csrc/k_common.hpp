@@ -1,0 +1,70 @@
+// k_common.hpp — device helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace plk {
+
+// Accumulator type: FP64 vectors accumulate in FP64, FP32 vectors (FP32 or
+// BF16 stored values) accumulate in FP32.
+template <class X> struct acc;
+template <> struct acc<double> { using type = double; };
+template <> struct acc<float> { using type = float; };
+template <class X> using acc_t = typename acc<X>::type;
+
+// Streaming loads of matrix data: non-temporal, so the value/index streams do
+// not evict the gathered x (or B) lines from L2.
+__device__ __forceinline__ double ldv(const double *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+__device__ __forceinline__ float ldv(const float *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+// BF16 storage: the upper 16 bits of an FP32 word.
+__device__ __forceinline__ float ldv(const uint16_t *p, int64_t i) {
+  const uint32_t u = __builtin_nontemporal_load(p + i);
+  return __uint_as_float(u << 16);
+}
+__device__ __forceinline__ int32_t ldi(const int32_t *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+__device__ __forceinline__ uint32_t ldo(const uint8_t *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+__device__ __forceinline__ uint32_t ldo(const uint16_t *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+__device__ __forceinline__ uint32_t ldo(const uint32_t *p, int64_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup b runs on XCD b % 8).  Remap the linear workgroup id so each XCD
+// works on one contiguous range of the problem: neighbouring rows then share
+// x (and B) lines in that XCD's L2 instead of every XCD fetching them.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t grid) {
+  const uint32_t x = b & 7u, i = b >> 3, q = grid >> 3, r = grid & 7u;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// Sum over aligned groups of `width` lanes (width a power of two <= 64);
+// every lane of the wave must call it.
+template <class T> __device__ __forceinline__ T group_sum(T v, int width) {
+  for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+template <int W, class T> __device__ __forceinline__ T group_sum_c(T v) {
+#pragma unroll
+  for (int o = W >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+constexpr int kBlock = 256;  // threads per workgroup (4 waves of 64)
+
+inline unsigned grid_for(int64_t threads) {
+  return unsigned((threads + kBlock - 1) / kBlock);
+}
+
+}  // namespace plk

@@ -1,0 +1,245 @@
+// k_select.hip — GPU kernels of the learned format selector.
+//
+//  * Matrix statistics for the 12 selector features, reduced without atomics
+//    (per-workgroup partials in LDS, then one workgroup folds the partials).
+//  * The 12 -> 64 -> 64 -> K MLP in FP32: forward, masked-MSE backward with
+//    per-weight deterministic reductions over samples, and Adam.
+#include "k_common.hpp"
+#include "plsparse.hpp"
+
+namespace plk {
+
+constexpr int kStats = 8;  // sum len, sum len^2, max len, empty rows,
+                           // sum spread, max |col-row|, 4x4 blocks, 8x8 blocks
+
+// Distinct b-wide column blocks touched by rows [rb*b, rb*b+b) (rows with
+// ascending columns; unsorted rows over-count).
+template <int B>
+__device__ int64_t distinct_blocks(const int32_t *rp, const int32_t *ci, int32_t m, int32_t rb) {
+  int32_t pos[B], end[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const int32_t r = rb * B + i;
+    pos[i] = r < m ? rp[r] : 0;
+    end[i] = r < m ? rp[r + 1] : 0;
+  }
+  int64_t count = 0;
+  while (true) {
+    int32_t best = INT32_MAX;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (pos[i] < end[i]) best = min(best, ci[pos[i]] / B);
+    if (best == INT32_MAX) break;
+    ++count;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      while (pos[i] < end[i] && ci[pos[i]] / B == best) ++pos[i];
+  }
+  return count;
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci,
+                                                      double *__restrict__ partial) {
+  __shared__ double red[kStats][kBlock];
+  const int tid = threadIdx.x;
+  const int64_t r = int64_t(blockIdx.x) * kBlock + tid;
+  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < m) {
+    const int32_t s = rp[r], e = rp[r + 1], len = e - s;
+    v[0] = len;
+    v[1] = double(len) * len;
+    v[2] = len;
+    v[3] = len == 0;
+    if (len > 0) {
+      int32_t mn = INT32_MAX, mx = INT32_MIN;
+      for (int32_t j = s; j < e; ++j) {
+        mn = min(mn, ci[j]);
+        mx = max(mx, ci[j]);
+      }
+      v[4] = double(mx) - mn + 1;
+      v[5] = fmax(fabs(double(mx) - double(r)), fabs(double(mn) - double(r)));
+    }
+  }
+  if (r < (int64_t(m) + 3) / 4) v[6] = double(distinct_blocks<4>(rp, ci, m, int32_t(r)));
+  if (r < (int64_t(m) + 7) / 8) v[7] = double(distinct_blocks<8>(rp, ci, m, int32_t(r)));
+#pragma unroll
+  for (int k = 0; k < kStats; ++k) red[k][tid] = v[k];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+#pragma unroll
+      for (int k = 0; k < kStats; ++k) {
+        const bool is_max = k == 2 || k == 5;
+        red[k][tid] = is_max ? fmax(red[k][tid], red[k][tid + o]) : red[k][tid] + red[k][tid + o];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < kStats) partial[int64_t(blockIdx.x) * kStats + tid] = red[tid][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fold_stats(int64_t nparts, const double *partial,
+                                                       double *out) {
+  __shared__ double red[kStats][kBlock];
+  const int tid = threadIdx.x;
+  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t p = tid; p < nparts; p += kBlock)
+#pragma unroll
+    for (int k = 0; k < kStats; ++k) {
+      const double x = partial[p * kStats + k];
+      v[k] = (k == 2 || k == 5) ? fmax(v[k], x) : v[k] + x;
+    }
+#pragma unroll
+  for (int k = 0; k < kStats; ++k) red[k][tid] = v[k];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (tid < o)
+#pragma unroll
+      for (int k = 0; k < kStats; ++k)
+        red[k][tid] = (k == 2 || k == 5) ? fmax(red[k][tid], red[k][tid + o])
+                                         : red[k][tid] + red[k][tid + o];
+    __syncthreads();
+  }
+  if (tid < kStats) out[tid] = red[tid][0];
+}
+
+hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s) {
+  const unsigned grid = grid_for(A.m);
+  if (grid == 0) return hipMemsetAsync(stats_dev, 0, sizeof(double) * kStats, s);
+  pls::DevBuf part;
+  hipError_t e = part.alloc(sizeof(double) * kStats * grid);
+  if (e != hipSuccess) return e;
+  k_row_stats<<<grid, kBlock, 0, s>>>(A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), part.as<double>());
+  k_fold_stats<<<1, kBlock, 0, s>>>(grid, part.as<double>(), stats_dev);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);  // `part` is freed on return
+}
+
+// ---------------------------------------------------------------------------
+// MLP.  Parameters flat: W1[I0 x H] b1[H] W2[H x H] b2[H] W3[H x K] b3[K],
+// W stored input-major (W[i * out + o]).
+
+// out[s, o] = act(b[o] + sum_i in[s, i] * W[i, o])
+__global__ __launch_bounds__(kBlock) void k_dense_fwd(int n, int I, int O, const float *in,
+                                                      const float *W, const float *b, float *out,
+                                                      bool relu) {
+  const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= int64_t(n) * O) return;
+  const int s = int(t / O), o = int(t % O);
+  float acc = b[o];
+  for (int i = 0; i < I; ++i) acc = fmaf(in[int64_t(s) * I + i], W[int64_t(i) * O + o], acc);
+  out[t] = relu ? fmaxf(acc, 0.f) : acc;
+}
+
+// Masked MSE: loss = sum M (out - T)^2 / max(1, sum M); one workgroup.
+__global__ __launch_bounds__(kBlock) void k_mse(int64_t total, const float *out, const float *T,
+                                                const float *M, float *loss_cnt) {
+  __shared__ float rl[kBlock], rc[kBlock];
+  float l = 0.f, c = 0.f;
+  for (int64_t i = threadIdx.x; i < total; i += kBlock) {
+    const float d = out[i] - T[i];
+    l += M[i] * d * d;
+    c += M[i];
+  }
+  rl[threadIdx.x] = l;
+  rc[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      rl[threadIdx.x] += rl[threadIdx.x + o];
+      rc[threadIdx.x] += rc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    loss_cnt[1] = fmaxf(rc[0], 1.f);
+    loss_cnt[0] = rl[0] / loss_cnt[1];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mse_grad(int64_t total, const float *out,
+                                                     const float *T, const float *M,
+                                                     const float *loss_cnt, float *d) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < total) d[i] = 2.f * M[i] * (out[i] - T[i]) / loss_cnt[1];
+}
+
+// d_in[s, i] = (sum_o d_out[s, o] * W[i, o]) * (act[s, i] > 0)
+__global__ __launch_bounds__(kBlock) void k_dense_bwd_act(int n, int I, int O, const float *dout,
+                                                          const float *W, const float *act,
+                                                          float *din) {
+  const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= int64_t(n) * I) return;
+  const int s = int(t / I), i = int(t % I);
+  float acc = 0.f;
+  for (int o = 0; o < O; ++o) acc = fmaf(dout[int64_t(s) * O + o], W[int64_t(i) * O + o], acc);
+  din[t] = act[t] > 0.f ? acc : 0.f;
+}
+
+// gW[i, o] = sum_s in[s, i] * dout[s, o];  gb[o] = sum_s dout[s, o]
+__global__ __launch_bounds__(kBlock) void k_dense_wgrad(int n, int I, int O, const float *in,
+                                                        const float *dout, float *gW, float *gb) {
+  const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= int64_t(I + 1) * O) return;
+  const int i = int(t / O), o = int(t % O);
+  float acc = 0.f;
+  if (i < I) {
+    for (int s = 0; s < n; ++s) acc = fmaf(in[int64_t(s) * I + i], dout[int64_t(s) * O + o], acc);
+    gW[int64_t(i) * O + o] = acc;
+  } else {
+    for (int s = 0; s < n; ++s) acc += dout[int64_t(s) * O + o];
+    gb[o] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adam(int64_t np, float *p, const float *g, float *m,
+                                                 float *v, float lr, float b1, float b2, float eps,
+                                                 float c1, float c2) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= np) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+hipError_t mlp_forward(const float *P, const float *X, int n, int I, int H, int K, float *h1,
+                       float *h2, float *out, hipStream_t s) {
+  const float *W1 = P, *b1 = W1 + I * H, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H,
+              *b3 = W3 + H * K;
+  k_dense_fwd<<<grid_for(int64_t(n) * H), kBlock, 0, s>>>(n, I, H, X, W1, b1, h1, true);
+  k_dense_fwd<<<grid_for(int64_t(n) * H), kBlock, 0, s>>>(n, H, H, h1, W2, b2, h2, true);
+  k_dense_fwd<<<grid_for(int64_t(n) * K), kBlock, 0, s>>>(n, H, K, h2, W3, b3, out, false);
+  return hipGetLastError();
+}
+
+hipError_t mlp_backward(const float *P, const float *X, int n, int I, int H, int K,
+                        const float *h1, const float *h2, const float *out, const float *T,
+                        const float *M, float *G, float *loss_cnt, float *dout, float *dh2,
+                        float *dh1, hipStream_t s) {
+  const float *W2 = P + I * H + H, *W3 = W2 + H * H + H;
+  float *gW1 = G, *gb1 = gW1 + I * H, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H,
+        *gb3 = gW3 + H * K;
+  const int64_t total = int64_t(n) * K;
+  k_mse<<<1, kBlock, 0, s>>>(total, out, T, M, loss_cnt);
+  k_mse_grad<<<grid_for(total), kBlock, 0, s>>>(total, out, T, M, loss_cnt, dout);
+  k_dense_wgrad<<<grid_for(int64_t(H + 1) * K), kBlock, 0, s>>>(n, H, K, h2, dout, gW3, gb3);
+  k_dense_bwd_act<<<grid_for(int64_t(n) * H), kBlock, 0, s>>>(n, H, K, dout, W3, h2, dh2);
+  k_dense_wgrad<<<grid_for(int64_t(H + 1) * H), kBlock, 0, s>>>(n, H, H, h1, dh2, gW2, gb2);
+  k_dense_bwd_act<<<grid_for(int64_t(n) * H), kBlock, 0, s>>>(n, H, H, dh2, W2, h1, dh1);
+  k_dense_wgrad<<<grid_for(int64_t(I + 1) * H), kBlock, 0, s>>>(n, I, H, X, dh1, gW1, gb1);
+  return hipGetLastError();
+}
+
+hipError_t adam_step(float *P, const float *G, float *m, float *v, int64_t np, int step, float lr,
+                     float b1, float b2, float eps, hipStream_t s) {
+  const float c1 = 1.f - powf(b1, float(step)), c2 = 1.f - powf(b2, float(step));
+  k_adam<<<grid_for(np), kBlock, 0, s>>>(np, P, G, m, v, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace plk

@@ -1,0 +1,464 @@
+// k_spmv.hip — SpMV kernels for gfx950 (CDNA4, wave64).
+//
+// All kernels compute y = alpha*A*x + beta*y with the reference's edge rule
+// (src/matvec.cpp:66-98, src/compressed_sparse_rows.cpp:1134-1137): with
+// beta == 0 the old y is never read, so NaN/Inf in y do not propagate.  Every
+// output element is reduced inside one workgroup with cross-lane (DPP /
+// ds_swizzle) shuffles and LDS; there are no atomics.  Matrix streams use
+// non-temporal loads; x is gathered through L2/MALL.
+#include "k_common.hpp"
+#include "plsparse.hpp"
+
+namespace plk {
+
+// ---------------------------------------------------------------------------
+// CSR-vector: L lanes per row (L a power of two).  Counterpart of the
+// reference's row-parallel kernels K1/K2 (src/compressed_sparse_rows.cpp:
+// 1114-1138, 1215-1321): same per-row result, the row's products split over L
+// lanes and combined by a shuffle tree instead of two interleaved partials.
+template <int L, class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_csr_vector(
+    int32_t m, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const V *__restrict__ val, const X *__restrict__ x, X *__restrict__ y,
+    acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  const int64_t t = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  const int64_t row = t / L;
+  const int lane = threadIdx.x & (L - 1);
+  A s0 = 0, s1 = 0;
+  if (row < m) {
+    const int32_t e = rp[row + 1];
+    int32_t j = rp[row] + lane;
+    for (; j + L < e; j += 2 * L) {
+      s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
+      s1 += A(ldv(val, j + L)) * A(x[ldi(ci, j + L)]);
+    }
+    if (j < e) s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
+  }
+  A s = group_sum_c<L>(s0 + s1);
+  if (row < m && lane == 0) {
+    A r = alpha * s;
+    if (BETA) r += beta * A(y[row]);
+    y[row] = X(r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CSR-adaptive: one workgroup per row block.  The host cuts the rows into
+// blocks holding at most kAdaNnz non-zeros (a row longer than that is a block
+// of its own), the GPU counterpart of the reference's nnz-balanced thread
+// split (gen_parallel_decomp_mv, src/compressed_sparse_rows.cpp:427-506).
+// Multi-row blocks stream their contiguous value/index range with fully
+// coalesced loads, stage the products in LDS and reduce each row with a
+// power-of-two lane group; single long rows are reduced across the whole
+// workgroup.  Rows split between threads in the reference (combined there
+// with atomics, src/compressed_sparse_rows.cpp:1146-1203) never occur: every
+// row is finished by exactly one workgroup.
+constexpr int kAdaNnz = 2048;
+
+template <class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_csr_adaptive(
+    const int32_t *__restrict__ blocks, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const V *__restrict__ val,
+    const X *__restrict__ x, X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  __shared__ A prod[kAdaNnz];
+  __shared__ A part[kBlock / 64];
+  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int32_t r0 = blocks[b], r1 = blocks[b + 1];
+  const int32_t e0 = rp[r0], e1 = rp[r1];
+  const int32_t nnz = e1 - e0, nrows = r1 - r0;
+
+  if (nnz > kAdaNnz) {  // one long row (nrows == 1 by construction)
+    A s0 = 0, s1 = 0;
+    int32_t j = e0 + tid;
+    for (; j + kBlock < e1; j += 2 * kBlock) {
+      s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
+      s1 += A(ldv(val, j + kBlock)) * A(x[ldi(ci, j + kBlock)]);
+    }
+    if (j < e1) s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
+    A s = group_sum_c<64>(s0 + s1);
+    if ((tid & 63) == 0) part[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+      A tot = (part[0] + part[1]) + (part[2] + part[3]);
+      A r = alpha * tot;
+      if (BETA) r += beta * A(y[r0]);
+      y[r0] = X(r);
+    }
+    return;
+  }
+
+#pragma unroll
+  for (int k = 0; k < kAdaNnz / kBlock; ++k) {
+    const int q = tid + k * kBlock;
+    if (q < nnz) prod[q] = A(ldv(val, e0 + q)) * A(x[ldi(ci, e0 + q)]);
+  }
+  __syncthreads();
+
+  if (nrows < kBlock) {
+    int lg = 6;  // lanes per row = 2^lg, the widest group that covers all rows
+    while ((nrows << lg) > kBlock) --lg;
+    const int g = tid >> lg, lane = tid & ((1 << lg) - 1);
+    A s = 0;
+    int32_t rs = 0, re = 0;
+    if (g < nrows) {
+      rs = rp[r0 + g] - e0;
+      re = rp[r0 + g + 1] - e0;
+      for (int32_t q = rs + lane; q < re; q += (1 << lg)) s += prod[q];
+    }
+    s = group_sum(s, 1 << lg);
+    if (g < nrows && lane == 0) {
+      A r = alpha * s;
+      if (BETA) r += beta * A(y[r0 + g]);
+      y[r0 + g] = X(r);
+    }
+  } else {
+    for (int32_t g = tid; g < nrows; g += kBlock) {
+      const int32_t rs = rp[r0 + g] - e0, re = rp[r0 + g + 1] - e0;
+      A s = 0;
+      for (int32_t q = rs; q < re; ++q) s += prod[q];
+      A r = alpha * s;
+      if (BETA) r += beta * A(y[r0 + g]);
+      y[r0 + g] = X(r);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SELL-C-sigma: one lane per (permuted) row, C consecutive lanes per chunk
+// (C = 64: a wave per chunk; C = 32: two chunks per wave), slices read
+// column-major so every slot load of a chunk is one contiguous C-element
+// access.  Same arithmetic as the reference's generic kernel and NEON kernels
+// (src/sell_c_sigma.cpp:719-797, src/asm/neon/d_spmv_scs_kernels.S.mako):
+// w[r] += vals[k]*x[offset[k] + col_min[chunk]], then
+// y[perm[r]] = alpha*w + beta*y (PERM) or y[chunk*C + r] (sigma = 0).  The
+// partial last chunk is masked by perm < 0 / row >= m instead of the
+// reference's separate tidy-up pass (src/d_opt_sell_c_sigma.cpp:43-139).
+template <int CT, class OFF, class V, class X, bool BETA, bool PERM>
+__global__ __launch_bounds__(kBlock) void k_sell(
+    int64_t nC, int32_t m, int32_t Crt, const int64_t *__restrict__ cs,
+    const int32_t *__restrict__ cl, const int32_t *__restrict__ cmin,
+    const OFF *__restrict__ offs, const V *__restrict__ val,
+    const int32_t *__restrict__ perm, const X *__restrict__ x,
+    X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  const int C = CT ? CT : Crt;
+  const int64_t t = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  const int64_t c = t / C;
+  const int r = int(t - c * C);
+  if (c >= nC) return;
+  const int64_t base = cs[c] + r;
+  const int32_t w = cl[c];
+  const X *xm = x + cmin[c];
+  A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int32_t j = 0;
+  for (; j + 3 < w; j += 4) {
+    const int64_t k = base + int64_t(j) * C;
+    s0 += A(ldv(val, k)) * A(xm[ldo(offs, k)]);
+    s1 += A(ldv(val, k + C)) * A(xm[ldo(offs, k + C)]);
+    s2 += A(ldv(val, k + 2 * C)) * A(xm[ldo(offs, k + 2 * C)]);
+    s3 += A(ldv(val, k + 3 * C)) * A(xm[ldo(offs, k + 3 * C)]);
+  }
+  for (; j < w; ++j) {
+    const int64_t k = base + int64_t(j) * C;
+    s0 += A(ldv(val, k)) * A(xm[ldo(offs, k)]);
+  }
+  const A s = (s0 + s1) + (s2 + s3);
+  const int64_t pr = c * C + r;
+  const int32_t row = PERM ? perm[pr] : int32_t(pr);
+  if (PERM ? row >= 0 : pr < m) {
+    A res = alpha * s;
+    if (BETA) res += beta * A(y[row]);
+    y[row] = X(res);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BSR, 4x4 and 8x8 blocks: one wave per block row.  b = 4: the 64 lanes hold
+// four consecutive blocks (16 values each) so one step reads 512 contiguous
+// bytes of FP64 values; b = 8: one block (64 values) per step.  Lane (i, j)
+// multiplies value (i, j) by x[col*b + j]; the j and block-slot partials are
+// folded with shuffles, leaving row i's sum in one lane.  Result rule of the
+// reference's generic and vanilla kernels (src/block_sparse_rows.cpp:458-727):
+// y = beta*y (or 0) + alpha*sum.
+template <int B, bool COLMAJ, class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_bsr_wave(
+    int32_t mb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const V *__restrict__ val, const X *__restrict__ x, X *__restrict__ y,
+    acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  constexpr int BB = B * B;
+  constexpr int PER = 64 / BB;  // blocks per wave step
+  const int64_t wave = (int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int slot = lane / BB, e = lane % BB;
+  const int i = COLMAJ ? e % B : e / B;
+  const int j = COLMAJ ? e / B : e % B;
+  if (wave >= mb) return;
+  const int32_t k0 = rp[wave], k1 = rp[wave + 1];
+  A s = 0;
+  for (int32_t k = k0 + slot; k < k1; k += PER)
+    s += A(ldv(val, int64_t(k) * BB + e)) * A(x[int64_t(ldi(ci, k)) * B + j]);
+  // fold j (lanes differing in the j bits) and the block slots
+  if (COLMAJ) {
+#pragma unroll
+    for (int o = B; o < 64; o <<= 1) s += __shfl_xor(s, o);
+  } else {
+#pragma unroll
+    for (int o = 1; o < B; o <<= 1) s += __shfl_xor(s, o);
+#pragma unroll
+    for (int o = BB; o < 64; o <<= 1) s += __shfl_xor(s, o);
+  }
+  const bool owner = COLMAJ ? lane < B : (slot == 0 && j == 0);
+  if (owner) {
+    const int64_t row = int64_t(wave) * B + i;
+    A r = alpha * s;
+    if (BETA) r += beta * A(y[row]);
+    y[row] = X(r);
+  }
+}
+
+// Any block size: one thread per matrix row.
+template <class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_bsr_generic(
+    int32_t mb, int32_t b, bool colmaj, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const V *__restrict__ val,
+    const X *__restrict__ x, X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  const int64_t row = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  if (row >= int64_t(mb) * b) return;
+  const int32_t rb = int32_t(row / b), i = int32_t(row % b);
+  const int64_t bb = int64_t(b) * b;
+  A s = 0;
+  for (int32_t k = rp[rb]; k < rp[rb + 1]; ++k) {
+    const int64_t xc = int64_t(ci[k]) * b;
+    for (int32_t j = 0; j < b; ++j)
+      s += A(ldv(val, k * bb + (colmaj ? i + int64_t(b) * j : int64_t(i) * b + j))) * A(x[xc + j]);
+  }
+  A r = alpha * s;
+  if (BETA) r += beta * A(y[row]);
+  y[row] = X(r);
+}
+
+// ---------------------------------------------------------------------------
+// Vector epilogues of the exec shortcuts (src/matvec.cpp:76-85).
+template <class X>
+__global__ __launch_bounds__(kBlock) void k_scale(int64_t n, acc_t<X> beta, X *y) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) y[i] = beta == acc_t<X>(0) ? X(0) : X(beta * acc_t<X>(y[i]));
+}
+template <class X>
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, acc_t<X> alpha, const X *x,
+                                                  acc_t<X> beta, X *y) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) {
+    acc_t<X> r = alpha * acc_t<X>(x[i]);
+    if (beta != acc_t<X>(0)) r += beta * acc_t<X>(y[i]);
+    y[i] = X(r);
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_f32_to_bf16(const float *src, uint16_t *dst,
+                                                        int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t u = __float_as_uint(src[i]);
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    dst[i] = uint16_t((u >> 16) | 0x40u);  // quiet NaN
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even
+    dst[i] = uint16_t(u >> 16);
+  }
+}
+
+// ===========================================================================
+// Host launchers
+namespace {
+
+template <class X> struct Scal {
+  static acc_t<X> of(double v) { return acc_t<X>(v); }
+};
+
+template <class F> hipError_t with_types(pls::VStore vs, bool dbl, F &&f) {
+  if (dbl) {
+    if (vs != pls::VStore::F64) return hipErrorInvalidValue;
+    return f((const double *)nullptr, (double *)nullptr);
+  }
+  if (vs == pls::VStore::F32) return f((const float *)nullptr, (float *)nullptr);
+  if (vs == pls::VStore::BF16) return f((const uint16_t *)nullptr, (float *)nullptr);
+  return hipErrorInvalidValue;
+}
+
+template <int L, class V, class X>
+void launch_vec(const pls::DevCsr &A, double alpha, const void *x, double beta, void *y,
+                hipStream_t s) {
+  const unsigned grid = grid_for(int64_t(A.m) * L);
+  if (grid == 0) return;
+  if (beta != 0.0)
+    k_csr_vector<L, V, X, true><<<grid, kBlock, 0, s>>>(
+        A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)x, (X *)y,
+        acc_t<X>(alpha), acc_t<X>(beta));
+  else
+    k_csr_vector<L, V, X, false><<<grid, kBlock, 0, s>>>(
+        A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)x, (X *)y,
+        acc_t<X>(alpha), acc_t<X>(0));
+}
+
+}  // namespace
+
+hipError_t csr_vector(const pls::DevCsr &A, bool dbl, double alpha, const void *x,
+                      double beta, void *y, hipStream_t s) {
+  return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    switch (A.vec_lanes) {
+      case 1: launch_vec<1, V, X>(A, alpha, x, beta, y, s); break;
+      case 2: launch_vec<2, V, X>(A, alpha, x, beta, y, s); break;
+      case 4: launch_vec<4, V, X>(A, alpha, x, beta, y, s); break;
+      case 8: launch_vec<8, V, X>(A, alpha, x, beta, y, s); break;
+      case 16: launch_vec<16, V, X>(A, alpha, x, beta, y, s); break;
+      case 32: launch_vec<32, V, X>(A, alpha, x, beta, y, s); break;
+      default: launch_vec<64, V, X>(A, alpha, x, beta, y, s); break;
+    }
+    return hipGetLastError();
+  });
+}
+
+hipError_t csr_adaptive(const pls::DevCsr &A, bool dbl, double alpha, const void *x,
+                        double beta, void *y, hipStream_t s) {
+  if (A.nblocks == 0) return hipSuccess;
+  return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    if (beta != 0.0)
+      k_csr_adaptive<V, X, true><<<A.nblocks, kBlock, 0, s>>>(
+          A.blocks.as<int32_t>(), A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)x, (X *)y, acc_t<X>(alpha), acc_t<X>(beta));
+    else
+      k_csr_adaptive<V, X, false><<<A.nblocks, kBlock, 0, s>>>(
+          A.blocks.as<int32_t>(), A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)x, (X *)y, acc_t<X>(alpha), acc_t<X>(0));
+    return hipGetLastError();
+  });
+}
+
+namespace {
+template <int CT, class OFF, class V, class X>
+void launch_sell(const pls::DevSell &A, double alpha, const void *x, double beta, void *y,
+                 hipStream_t s) {
+  const unsigned grid = grid_for(A.nC * A.C);
+  if (grid == 0) return;
+  const bool perm = A.sigma != 0;
+  const auto *cs = A.cs.as<int64_t>();
+  const auto *cl = A.cl.as<int32_t>();
+  const auto *cm = A.cmin.as<int32_t>();
+  const auto *of = A.offs.as<OFF>();
+  const auto *vv = A.vals.as<V>();
+  const auto *pm = A.perm.as<int32_t>();
+  const acc_t<X> al = acc_t<X>(alpha), be = acc_t<X>(beta);
+  if (beta != 0.0) {
+    if (perm)
+      k_sell<CT, OFF, V, X, true, true><<<grid, kBlock, 0, s>>>(A.nC, A.m, A.C, cs, cl, cm, of, vv, pm, (const X *)x, (X *)y, al, be);
+    else
+      k_sell<CT, OFF, V, X, true, false><<<grid, kBlock, 0, s>>>(A.nC, A.m, A.C, cs, cl, cm, of, vv, pm, (const X *)x, (X *)y, al, be);
+  } else {
+    if (perm)
+      k_sell<CT, OFF, V, X, false, true><<<grid, kBlock, 0, s>>>(A.nC, A.m, A.C, cs, cl, cm, of, vv, pm, (const X *)x, (X *)y, al, be);
+    else
+      k_sell<CT, OFF, V, X, false, false><<<grid, kBlock, 0, s>>>(A.nC, A.m, A.C, cs, cl, cm, of, vv, pm, (const X *)x, (X *)y, al, be);
+  }
+}
+template <class OFF, class V, class X>
+void launch_sell_c(const pls::DevSell &A, double alpha, const void *x, double beta, void *y,
+                   hipStream_t s) {
+  if (A.C == 64) launch_sell<64, OFF, V, X>(A, alpha, x, beta, y, s);
+  else if (A.C == 32) launch_sell<32, OFF, V, X>(A, alpha, x, beta, y, s);
+  else launch_sell<0, OFF, V, X>(A, alpha, x, beta, y, s);
+}
+}  // namespace
+
+hipError_t sell_spmv(const pls::DevSell &A, bool dbl, double alpha, const void *x,
+                     double beta, void *y, hipStream_t s) {
+  return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    switch (A.col_bytes) {
+      case 1: launch_sell_c<uint8_t, V, X>(A, alpha, x, beta, y, s); break;
+      case 2: launch_sell_c<uint16_t, V, X>(A, alpha, x, beta, y, s); break;
+      default: launch_sell_c<uint32_t, V, X>(A, alpha, x, beta, y, s); break;
+    }
+    return hipGetLastError();
+  });
+}
+
+namespace {
+template <int B, bool CM, class V, class X>
+void launch_bsr_wave(const pls::DevBsr &A, double alpha, const void *x, double beta, void *y,
+                     hipStream_t s) {
+  const unsigned grid = grid_for(int64_t(A.mb) * 64);
+  if (grid == 0) return;
+  if (beta != 0.0)
+    k_bsr_wave<B, CM, V, X, true><<<grid, kBlock, 0, s>>>(
+        A.mb, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)x, (X *)y,
+        acc_t<X>(alpha), acc_t<X>(beta));
+  else
+    k_bsr_wave<B, CM, V, X, false><<<grid, kBlock, 0, s>>>(
+        A.mb, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)x, (X *)y,
+        acc_t<X>(alpha), acc_t<X>(0));
+}
+}  // namespace
+
+hipError_t bsr_spmv(const pls::DevBsr &A, bool dbl, double alpha, const void *x,
+                    double beta, void *y, hipStream_t s) {
+  return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    const bool cm = A.layout == PERFLIBS_COL_MAJOR;
+    if (A.b == 4) {
+      if (cm) launch_bsr_wave<4, true, V, X>(A, alpha, x, beta, y, s);
+      else launch_bsr_wave<4, false, V, X>(A, alpha, x, beta, y, s);
+    } else if (A.b == 8) {
+      if (cm) launch_bsr_wave<8, true, V, X>(A, alpha, x, beta, y, s);
+      else launch_bsr_wave<8, false, V, X>(A, alpha, x, beta, y, s);
+    } else {
+      const unsigned grid = grid_for(int64_t(A.mb) * A.b);
+      if (grid == 0) return hipSuccess;
+      if (beta != 0.0)
+        k_bsr_generic<V, X, true><<<grid, kBlock, 0, s>>>(
+            A.mb, A.b, cm, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+            (const X *)x, (X *)y, acc_t<X>(alpha), acc_t<X>(beta));
+      else
+        k_bsr_generic<V, X, false><<<grid, kBlock, 0, s>>>(
+            A.mb, A.b, cm, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+            (const X *)x, (X *)y, acc_t<X>(alpha), acc_t<X>(0));
+    }
+    return hipGetLastError();
+  });
+}
+
+hipError_t scale_vec(bool dbl, int64_t n, double beta, void *y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (dbl) k_scale<double><<<grid_for(n), kBlock, 0, s>>>(n, beta, (double *)y);
+  else k_scale<float><<<grid_for(n), kBlock, 0, s>>>(n, float(beta), (float *)y);
+  return hipGetLastError();
+}
+
+hipError_t axpby_vec(bool dbl, int64_t n, double alpha, const void *x, double beta,
+                     void *y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (dbl)
+    k_axpby<double><<<grid_for(n), kBlock, 0, s>>>(n, alpha, (const double *)x, beta, (double *)y);
+  else
+    k_axpby<float><<<grid_for(n), kBlock, 0, s>>>(n, float(alpha), (const float *)x, float(beta),
+                                                  (float *)y);
+  return hipGetLastError();
+}
+
+hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_f32_to_bf16<<<grid_for(n), kBlock, 0, s>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+}  // namespace plk

@@ -1,0 +1,312 @@
+// plsparse.hpp — internal declarations of libplsparse_amd.
+//
+// A handle (perflibs_spmat_top_t) keeps the matrix as the caller created it on
+// the host (CSR, BSR or dense, deep copy or NOCOPY view) plus at most one device
+// "plan": the HBM layout of the SpMV kernel variant chosen at optimize time
+// (CSR row blocks, SELL-C-sigma slices or BSR blocks).  Dense operands of SpMM
+// keep a device mirror with validity flags so results stay in HBM until
+// exported.  Every index array on the device is 0-based int32 except the
+// SELL-C-sigma chunk offsets (int64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "perflibs_sparse.h"
+#include "perflibs_sparse_amd.h"
+
+namespace pls {
+
+enum class Fmt { Null, Identity, Csr, Bsr, Scs, Dense };
+enum class DType { F32 = 1, F64 = 2 };
+// Storage type of matrix values on the device.
+enum class VStore { F64 = 0, F32 = 1, BF16 = 2 };
+
+struct ErrorInfo {
+  int type = 0;
+  int64_t code = 0;
+  std::string msg;
+};
+
+// Move-only device allocation (hipMalloc / hipFree).
+class DevBuf {
+ public:
+  DevBuf() = default;
+  ~DevBuf() { reset(); }
+  DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { reset(); p_ = o.p_; bytes_ = o.bytes_; o.p_ = nullptr; o.bytes_ = 0; }
+    return *this;
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  hipError_t alloc(size_t bytes);
+  // Allocate and copy `bytes` from host memory.
+  hipError_t upload(const void *host, size_t bytes);
+  void reset();
+  void *get() const { return p_; }
+  template <class T> T *as() const { return static_cast<T *>(p_); }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Host CSR owned by (or viewed through) a handle; index base as created.
+struct HostCsr {
+  std::vector<int32_t> rp_own, ci_own;
+  std::vector<double> vd_own;
+  std::vector<float> vf_own;
+  const int32_t *rp = nullptr;
+  const int32_t *ci = nullptr;
+  const void *vals = nullptr;  // double* or float* by handle dtype
+  void point_to_own(DType dt) {
+    rp = rp_own.data();
+    ci = ci_own.data();
+    vals = dt == DType::F64 ? (const void *)vd_own.data() : (const void *)vf_own.data();
+  }
+};
+
+struct HostBsr {
+  int layout = PERFLIBS_ROW_MAJOR;
+  int32_t b = 0, mb = 0, nb = 0;
+  int64_t nnzb = 0;
+  std::vector<int32_t> rp_own, ci_own;
+  std::vector<double> vd_own;
+  std::vector<float> vf_own;
+  const int32_t *rp = nullptr;
+  const int32_t *ci = nullptr;
+  const void *vals = nullptr;
+};
+
+// Dense matrix object: host values (own copy or user view) and a device
+// mirror.  Exactly one of host/device may be stale.
+struct Dense {
+  int layout = PERFLIBS_COL_MAJOR;
+  int64_t m = 0, n = 0, lda = 0;
+  std::vector<double> vd_own;
+  std::vector<float> vf_own;
+  const void *user = nullptr;  // NOCOPY view
+  bool host_valid = true;
+  DevBuf dev;                 // same layout and lda as the host copy
+  bool dev_valid = false;
+  bool matrix_is_zero = false;
+};
+
+// Device CSR (0-based) with the row-block table of the adaptive kernel.
+struct DevCsr {
+  int32_t m = 0, n = 0;
+  int64_t nnz = 0;
+  VStore vs = VStore::F64;
+  DevBuf rp, ci, vals;
+  DevBuf blocks;  // int32 row starts of nnz-balanced row blocks (+ final m)
+  int32_t nblocks = 0;
+  int32_t vec_lanes = 8;  // lanes per row of the CSR-vector kernel
+};
+
+// Device SELL-C-sigma (layout of src/sell_c_sigma.hpp:43-104, NOTRANS).
+struct DevSell {
+  int32_t m = 0, n = 0, C = 0, sigma = 0;
+  int64_t nC = 0, padded = 0;
+  int32_t col_bytes = 4, n_full_chunks = 0, ntidyup = 0;
+  VStore vs = VStore::F64;
+  DevBuf cs;    // int64 [nC]
+  DevBuf cl;    // int32 [nC]
+  DevBuf perm;  // int32 [nC*C], -1 for padding rows
+  DevBuf cmin;  // int32 [nC]
+  DevBuf offs;  // uint8/16/32 [padded]
+  DevBuf vals;  // [padded]
+  // host copies kept for export, value update and layout tests
+  std::vector<int64_t> h_cs, h_cl, h_permd2in, h_in2permd, h_cmin;
+};
+
+struct DevBsr {
+  int layout = PERFLIBS_ROW_MAJOR;
+  int32_t b = 0, mb = 0, nb = 0;
+  int64_t nnzb = 0;
+  VStore vs = VStore::F64;
+  DevBuf rp, ci, vals;
+};
+
+// Which operand the device plan was built for.
+struct Plan {
+  int variant = -1;  // perflibs_amd_variant
+  int op = PERFLIBS_SPARSE_OPERATION_NOTRANS;
+  std::unique_ptr<DevCsr> csr;
+  std::unique_ptr<DevSell> sell;
+  std::unique_ptr<DevBsr> bsr;
+  void clear() { variant = -1; csr.reset(); sell.reset(); bsr.reset(); }
+};
+
+}  // namespace pls
+
+// The opaque handle type of perflibs_sparse.h.
+struct perflibs_spmat_top_t {
+  pls::DType dt = pls::DType::F64;
+  pls::Fmt fmt = pls::Fmt::Null;
+  int32_t m = 0, n = 0, index_base = 0;
+  int32_t nnz = 0;
+  bool nocopy = false;
+
+  int hint_structure = PERFLIBS_SPARSE_STRUCTURE_UNSTRUCTURED;
+  int hint_memory = PERFLIBS_SPARSE_MEMORY_ALLOCS;
+  int spmv_op = PERFLIBS_SPARSE_OPERATION_NOTRANS;
+  int spmv_inv = PERFLIBS_SPARSE_INVOCATIONS_MANY;
+  int spmm_op = PERFLIBS_SPARSE_OPERATION_NOTRANS;
+  int spmm_inv = PERFLIBS_SPARSE_INVOCATIONS_MANY;
+  int other_hints[8] = {PERFLIBS_SPARSE_OPERATION_NOTRANS, PERFLIBS_SPARSE_INVOCATIONS_MANY,
+                        PERFLIBS_SPARSE_SPMM_STRAT_UNSET, PERFLIBS_SPARSE_OPERATION_NOTRANS,
+                        PERFLIBS_SPARSE_OPERATION_NOTRANS, PERFLIBS_SPARSE_INVOCATIONS_MANY,
+                        PERFLIBS_SPARSE_SPSV_STRAT_UNSET, 0};
+
+  pls::ErrorInfo err;
+
+  pls::HostCsr csr;    // fmt Csr, and the source of fmt Scs
+  pls::HostBsr bsr;    // fmt Bsr
+  pls::Dense dense;    // fmt Dense
+  int32_t scs_C = 0, scs_sigma = 0;  // fmt Scs
+  int scs_op = PERFLIBS_SPARSE_OPERATION_NOTRANS;  // op the SELL object was built for
+
+  pls::Plan plan;      // device plan for SpMV / SpMM
+  int select_mode = -1;   // -1: learned when a model is loaded, else heuristic
+  int forced_variant = -1;
+  int forced_C = -1, forced_sigma = -1;
+  int precision = PERFLIBS_AMD_VALUES_NATIVE;
+
+  pls::DevBuf stage_x, stage_y;  // staging of host x / y
+};
+
+namespace pls {
+
+// ---- runtime (device.cpp) ----
+hipStream_t stream();
+void set_stream(hipStream_t s);
+bool is_device_ptr(const void *p);
+// Record a failure on the handle (may be null) and return status 2.
+perflibs_status_t hip_fail(perflibs_spmat_top_t *A, hipError_t e, const char *what);
+perflibs_status_t exec_fail(perflibs_spmat_top_t *A, const std::string &msg);
+perflibs_status_t param_fail(perflibs_spmat_top_t *A, int64_t code, const std::string &msg = "");
+void set_last_error(const std::string &msg);
+// Checks a usable HIP device exists (once per process).
+hipError_t ensure_device();
+
+// ---- host-side conversions (convert.cpp) ----
+// bsr2csr of src/compressed_sparse_rows.cpp:966-1056 (drops zeros, keeps base).
+void bsr_to_csr(const perflibs_spmat_top_t *A, HostCsr &out, int32_t &nnz_out);
+// CSR (index base b) -> 0-based CSR of the transpose.
+void csr_transpose0(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci,
+                    const void *vals, DType dt, std::vector<int32_t> &trp,
+                    std::vector<int32_t> &tci, std::vector<double> &tvd,
+                    std::vector<float> &tvf);
+// CSR (0-based) -> BSR (row-major blocks, explicit zeros), false if m or n
+// is not a multiple of b.
+bool csr_to_bsr0(int32_t m, int32_t n, int32_t b, const int32_t *rp,
+                 const int32_t *ci, const double *vd, const float *vf,
+                 std::vector<int32_t> &brp, std::vector<int32_t> &bci,
+                 std::vector<double> &bvd, std::vector<float> &bvf);
+
+// ---- SELL-C-sigma host build (sell_build.cpp) ----
+struct SellHost {
+  int32_t C = 0, sigma = 0;
+  int64_t nC = 0, padded = 0;
+  int32_t col_bytes = 4, n_full_chunks = 0, ntidyup = 0;
+  std::vector<int64_t> cs, cl, permd2in, in2permd, cmin;
+  std::vector<int32_t> chunk_min_nnz;
+};
+// Structure of csr2scs (src/sell_c_sigma.cpp:145-336); 0-based or 1-based CSR.
+bool sell_structure(int32_t m, const int32_t *rp, const int32_t *ci, int32_t C,
+                    int32_t sigma, SellHost &s);
+// Scatter of gen_data_vectors (src/sell_c_sigma.cpp:584-705) into typed
+// arrays: vals in the handle dtype, offsets at col_bytes bytes each.
+void sell_fill(const SellHost &s, int32_t m, const int32_t *rp, const int32_t *ci,
+               const void *vals, DType dt, std::vector<uint8_t> &vals_out,
+               std::vector<uint8_t> &offs_out);
+
+// ---- plans (plan.cpp) ----
+// Host CSR operand of op(A) (0-based) used to build any plan.
+struct Operand {
+  int32_t m = 0, n = 0;
+  std::vector<int32_t> rp, ci;
+  std::vector<double> vd;
+  std::vector<float> vf;
+  const int32_t *prp = nullptr, *pci = nullptr;
+  const void *pvals = nullptr;
+  int32_t base = 0;
+  // block fill nnz / (stored blocks * b^2), filled in by compute_features
+  mutable float fill4 = -1.f, fill8 = -1.f;
+};
+bool make_operand(const perflibs_spmat_top_t *A, int op, Operand &o);
+// The variant can run on this operand (dimensions).
+bool variant_applicable(const perflibs_spmat_top_t *A, const Operand &o, int variant);
+// The variant is worth considering during selection: applicable, and for BSR
+// a block fill of at least 1/4 (known from the features), so blocking never
+// inflates the stored values more than 4x.
+bool variant_candidate(const perflibs_spmat_top_t *A, const Operand &o, int variant);
+perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
+                             int variant);
+perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
+                                  int32_t C, int32_t sigma, int variant_tag);
+perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A);
+// Launch the current plan: y = alpha*A*x + beta*y on device vectors.
+perflibs_status_t launch_plan(perflibs_spmat_top_t *A, double alpha,
+                              const void *x, double beta, void *y);
+VStore value_store(const perflibs_spmat_top_t *A);
+
+// ---- selection (select.cpp) ----
+int pick_variant(perflibs_spmat_top_t *A, const Operand &o);
+perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o,
+                                   float *feat);
+int heuristic_variant(const float *feat, const Operand &o);
+perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op,
+                                int warmup, int reps, double *ms);
+
+}  // namespace pls
+
+// ---- kernel launchers (.hip files); all enqueue on pls::stream() ----
+namespace plk {
+using pls::VStore;
+// SpMV, FP64 or FP32 vectors (dbl selects), values stored as vs.
+hipError_t csr_vector(const pls::DevCsr &A, bool dbl, double alpha, const void *x,
+                      double beta, void *y, hipStream_t s);
+hipError_t csr_adaptive(const pls::DevCsr &A, bool dbl, double alpha, const void *x,
+                        double beta, void *y, hipStream_t s);
+hipError_t sell_spmv(const pls::DevSell &A, bool dbl, double alpha, const void *x,
+                     double beta, void *y, hipStream_t s);
+hipError_t bsr_spmv(const pls::DevBsr &A, bool dbl, double alpha, const void *x,
+                    double beta, void *y, hipStream_t s);
+// y = beta*y (beta != 0) or y = 0 (beta == 0), n elements.
+hipError_t scale_vec(bool dbl, int64_t n, double beta, void *y, hipStream_t s);
+// y = alpha*x + beta*y.
+hipError_t axpby_vec(bool dbl, int64_t n, double alpha, const void *x, double beta,
+                     void *y, hipStream_t s);
+// C(i,j) = alpha * sum A(i,p) B(p,j) + beta*C(i,j) (beta==0: C ignored) with
+// B(r,c) at B[r*bsr + c*bsc] and C(r,c) at C[r*csr + c*csc].
+hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha,
+                    const void *B, int64_t b_sr, int64_t b_sc, double beta, void *C,
+                    int64_t c_sr, int64_t c_sc, hipStream_t s);
+// Scale a strided m x n dense matrix by beta (beta == 0: set to zero).
+hipError_t scale_dense(bool dbl, int64_t m, int64_t n, double beta, void *C,
+                       int64_t sr, int64_t sc, hipStream_t s);
+// dst[i*ds] = src[i*ss], i < n.
+hipError_t strided_copy(bool dbl, int64_t n, const void *src, int64_t ss, void *dst, int64_t ds,
+                        hipStream_t s);
+// FP32 -> BF16 (round to nearest even), n values, device to device.
+hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s);
+// Row statistics of a 0-based device CSR for the selector features:
+// out[0..] partial sums reduced on the device, see select.cpp.
+hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s);
+// MLP of the selector (I inputs, H hidden, K outputs), FP32 device buffers.
+hipError_t mlp_forward(const float *P, const float *X, int n, int I, int H, int K, float *h1,
+                       float *h2, float *out, hipStream_t s);
+hipError_t mlp_backward(const float *P, const float *X, int n, int I, int H, int K,
+                        const float *h1, const float *h2, const float *out, const float *T,
+                        const float *M, float *G, float *loss_cnt, float *dout, float *dh2,
+                        float *dh1, hipStream_t s);
+hipError_t adam_step(float *P, const float *G, float *m, float *v, int64_t np, int step, float lr,
+                     float b1, float b2, float eps, hipStream_t s);
+}  // namespace plk

@@ -1,0 +1,268 @@
+// select.cpp — SpMV variant selection at optimize time.
+//
+// Replaces the reference's audition (src/matvec.cpp:189-518: SELL-C-sigma
+// candidates filtered on padding efficiency, then native CSR / vanilla / SELL
+// timed with a Welch t-test early stop) by four modes:
+//   EXHAUSTIVE — time every applicable variant on the GPU, keep the fastest
+//                (the judge of the other modes' regret);
+//   LEARNED    — a 12 -> 64 -> 64 -> K MLP on matrix features predicts
+//                log(time / best time) per variant, argmin wins;
+//   HEURISTIC  — fixed rules on the same features (the no-network baseline);
+//   FIXED_CSR  — always the CSR-vector kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "plsparse.hpp"
+
+namespace pls {
+
+namespace {
+constexpr int kF = PERFLIBS_AMD_NUM_FEATURES;
+constexpr int kH = PERFLIBS_AMD_MLP_HIDDEN;
+constexpr int kK = PERFLIBS_AMD_NUM_VARIANTS;
+constexpr size_t kNumParams = size_t(kF) * kH + kH + size_t(kH) * kH + kH + size_t(kH) * kK + kK;
+
+struct Model {
+  bool loaded = false;
+  DevBuf params;
+  float mean[kF] = {0}, stdv[kF] = {0};
+};
+Model g_model;
+}  // namespace
+
+perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, float *feat) {
+  // a CSR plan of the operand is the feature kernels' input
+  if (!A->plan.csr || A->plan.op != A->spmv_op) {
+    perflibs_status_t st = build_plan(A, o, A->spmv_op, PERFLIBS_AMD_CSR_VECTOR);
+    if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  }
+  const DevCsr *csr = A->plan.csr.get();
+  DevBuf stats;
+  hipError_t e = stats.alloc(8 * sizeof(double));
+  if (e == hipSuccess) e = plk::csr_row_stats(*csr, stats.as<double>(), stream());
+  double st[8] = {0};
+  if (e == hipSuccess) e = hipMemcpy(st, stats.get(), sizeof(st), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(A, e, "matrix statistics");
+  const double m = std::max(1, o.m), n = std::max(1, o.n);
+  const double nnz = st[0];
+  const double mean = nnz / m;
+  const double var = std::max(0.0, st[1] / m - mean * mean);
+  const double mx = st[2];
+  const double nonempty = std::max(1.0, m - st[3]);
+  feat[0] = float(std::log(m));
+  feat[1] = float(std::log1p(nnz));
+  feat[2] = float(nnz / n);
+  feat[3] = float(std::log1p(mean));
+  feat[4] = float(std::log1p(var));
+  feat[5] = float(std::log1p(mx));
+  feat[6] = float(mean > 0 ? std::log(mx / mean) : 0.0);
+  feat[7] = float(st[5] / n);
+  feat[8] = float(st[4] / nonempty / n);
+  feat[9] = float(st[6] > 0 ? nnz / (st[6] * 16.0) : 0.0);
+  feat[10] = float(st[7] > 0 ? nnz / (st[7] * 64.0) : 0.0);
+  feat[11] = float(st[3] / m);
+  o.fill4 = feat[9];
+  o.fill8 = feat[10];
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+// Hand rules on the same features: dense blocks -> BSR, strongly irregular
+// rows -> CSR-adaptive, otherwise SELL-C-sigma with a moderate sort window.
+int heuristic_variant(const float *f, const Operand &o) {
+  const double fill4 = f[9], fill8 = f[10], imbalance = f[6], mean = std::expm1(f[3]);
+  if (fill8 >= 0.9 && o.m % 8 == 0 && o.n % 8 == 0) return PERFLIBS_AMD_BSR_B8;
+  if (fill4 >= 0.9 && o.m % 4 == 0 && o.n % 4 == 0) return PERFLIBS_AMD_BSR_B4;
+  if (imbalance > std::log(8.0)) return PERFLIBS_AMD_CSR_ADAPTIVE;
+  return mean >= 24 ? PERFLIBS_AMD_SELL_C64_S1024 : PERFLIBS_AMD_SELL_C32_S1024;
+}
+
+perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op, int warmup,
+                                int reps, double *ms) {
+  hipError_t e = ensure_device();
+  if (e != hipSuccess) return hip_fail(A, e, "no usable HIP device");
+  const size_t es = A->dt == DType::F64 ? 8 : 4;
+  DevBuf x, y;
+  e = x.alloc(es * size_t(std::max(o.n, 1)));
+  if (e == hipSuccess) e = y.alloc(es * size_t(std::max(o.m, 1)));
+  if (e == hipSuccess) e = hipMemset(x.get(), 0, es * size_t(std::max(o.n, 1)));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  if (e != hipSuccess) return hip_fail(A, e, "timing setup");
+  if (o.fill4 < 0.f) {
+    float f[kF];
+    perflibs_status_t fs = compute_features(A, o, f);
+    if (fs != PERFLIBS_STATUS_SUCCESS) return fs;
+  }
+  perflibs_status_t status = PERFLIBS_STATUS_SUCCESS;
+  for (int v = 0; v < kK; ++v) {
+    ms[v] = -1.0;
+    if (!variant_candidate(A, o, v)) continue;
+    perflibs_status_t st = build_plan(A, o, op, v);
+    if (st != PERFLIBS_STATUS_SUCCESS) continue;
+    for (int w = 0; w < warmup && st == PERFLIBS_STATUS_SUCCESS; ++w)
+      st = launch_plan(A, 1.0, x.get(), 0.0, y.get());
+    e = hipEventRecord(t0, stream());
+    for (int r = 0; r < reps && st == PERFLIBS_STATUS_SUCCESS; ++r)
+      st = launch_plan(A, 1.0, x.get(), 0.0, y.get());
+    if (e == hipSuccess) e = hipEventRecord(t1, stream());
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    float el = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&el, t0, t1);
+    if (e != hipSuccess) {
+      status = hip_fail(A, e, "variant timing");
+      break;
+    }
+    if (st == PERFLIBS_STATUS_SUCCESS) ms[v] = double(el) / std::max(reps, 1);
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  A->plan.clear();
+  return status;
+}
+
+// MLP inference on the GPU: standardised features -> predicted log slowdown.
+static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const float *f) {
+  float xs[kF];
+  for (int i = 0; i < kF; ++i) xs[i] = (f[i] - g_model.mean[i]) / (g_model.stdv[i] > 0 ? g_model.stdv[i] : 1.f);
+  DevBuf X, h1, h2, out;
+  hipError_t e = X.upload(xs, sizeof(xs));
+  if (e == hipSuccess) e = h1.alloc(sizeof(float) * kH);
+  if (e == hipSuccess) e = h2.alloc(sizeof(float) * kH);
+  if (e == hipSuccess) e = out.alloc(sizeof(float) * kK);
+  if (e == hipSuccess)
+    e = plk::mlp_forward(g_model.params.as<float>(), X.as<float>(), 1, kF, kH, kK, h1.as<float>(),
+                         h2.as<float>(), out.as<float>(), stream());
+  float pred[kK];
+  if (e == hipSuccess) e = hipMemcpy(pred, out.get(), sizeof(pred), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return heuristic_variant(f, o);
+  int best = PERFLIBS_AMD_CSR_VECTOR;
+  float bv = 1e30f;
+  for (int v = 0; v < kK; ++v)
+    if (variant_candidate(A, o, v) && pred[v] < bv) {
+      bv = pred[v];
+      best = v;
+    }
+  return best;
+}
+
+int pick_variant(perflibs_spmat_top_t *A, const Operand &o) {
+  const int op = A->spmv_op;
+  if (A->forced_variant >= 0)
+    return variant_applicable(A, o, A->forced_variant) ? A->forced_variant : PERFLIBS_AMD_CSR_VECTOR;
+  int mode = A->select_mode;
+  if (mode < 0) mode = g_model.loaded ? PERFLIBS_AMD_SELECT_LEARNED : PERFLIBS_AMD_SELECT_HEURISTIC;
+  if (mode == PERFLIBS_AMD_SELECT_FIXED_CSR) return PERFLIBS_AMD_CSR_VECTOR;
+  if (mode == PERFLIBS_AMD_SELECT_EXHAUSTIVE) {
+    double ms[kK];
+    if (time_variants(A, o, op, 3, 10, ms) != PERFLIBS_STATUS_SUCCESS) return PERFLIBS_AMD_CSR_VECTOR;
+    int best = PERFLIBS_AMD_CSR_VECTOR;
+    for (int v = 0; v < kK; ++v)
+      if (ms[v] > 0 && (ms[best] <= 0 || ms[v] < ms[best])) best = v;
+    return best;
+  }
+  float f[kF];
+  if (compute_features(A, o, f) != PERFLIBS_STATUS_SUCCESS) return PERFLIBS_AMD_CSR_VECTOR;
+  if (mode == PERFLIBS_AMD_SELECT_LEARNED && g_model.loaded) return learned_variant(A, o, f);
+  return heuristic_variant(f, o);
+}
+
+}  // namespace pls
+
+extern "C" {
+
+size_t perflibs_amd_mlp_num_params(void) { return pls::kNumParams; }
+
+perflibs_status_t perflibs_amd_set_selector_model(const float *params, const float *feat_mean,
+                                                  const float *feat_std) {
+  if (!params || !feat_mean || !feat_std) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  hipError_t e = pls::ensure_device();
+  if (e == hipSuccess) e = pls::g_model.params.upload(params, sizeof(float) * pls::kNumParams);
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "selector model upload");
+  std::memcpy(pls::g_model.mean, feat_mean, sizeof(pls::g_model.mean));
+  std::memcpy(pls::g_model.stdv, feat_std, sizeof(pls::g_model.stdv));
+  pls::g_model.loaded = true;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_clear_selector_model(void) {
+  pls::g_model.loaded = false;
+  pls::g_model.params.reset();
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_features(perflibs_spmat_t A, float *features) {
+  if (!A || !features) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  hipError_t e = pls::ensure_device();
+  if (e != hipSuccess) return pls::hip_fail(A, e, "no usable HIP device");
+  pls::Operand o;
+  if (!pls::make_operand(A, A->spmv_op, o)) return pls::exec_fail(A, "no CSR view of this matrix");
+  return pls::compute_features(A, o, features);
+}
+
+perflibs_status_t perflibs_amd_time_variants(perflibs_spmat_t A, int warmup, int reps, double *ms) {
+  if (!A || !ms || reps <= 0 || warmup < 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  pls::Operand o;
+  if (!pls::make_operand(A, A->spmv_op, o)) return pls::exec_fail(A, "no CSR view of this matrix");
+  return pls::time_variants(A, o, A->spmv_op, warmup, reps, ms);
+}
+
+// ---- MLP primitives (device buffers) ----
+namespace {
+struct Scratch {
+  pls::DevBuf dout, dh2, dh1;
+  int cap = 0;
+};
+Scratch g_scratch;
+}  // namespace
+
+perflibs_status_t perflibs_amd_mlp_forward(const float *P, const float *X, int n, float *h1,
+                                           float *h2, float *out) {
+  if (!P || !X || !out || n <= 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  pls::DevBuf t1, t2;
+  hipError_t e = hipSuccess;
+  if (!h1) {
+    e = t1.alloc(sizeof(float) * size_t(n) * pls::kH);
+    h1 = t1.as<float>();
+  }
+  if (e == hipSuccess && !h2) {
+    e = t2.alloc(sizeof(float) * size_t(n) * pls::kH);
+    h2 = t2.as<float>();
+  }
+  if (e == hipSuccess) e = plk::mlp_forward(P, X, n, pls::kF, pls::kH, pls::kK, h1, h2, out, pls::stream());
+  if (e == hipSuccess && (t1.get() || t2.get())) e = hipStreamSynchronize(pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "MLP forward");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_mlp_backward(const float *P, const float *X, int n, const float *h1,
+                                            const float *h2, const float *out, const float *T,
+                                            const float *M, float *G, float *loss) {
+  if (!P || !X || !h1 || !h2 || !out || !T || !M || !G || !loss || n <= 0)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  hipError_t e = hipSuccess;
+  if (g_scratch.cap < n) {
+    e = g_scratch.dout.alloc(sizeof(float) * size_t(n) * pls::kK);
+    if (e == hipSuccess) e = g_scratch.dh2.alloc(sizeof(float) * size_t(n) * pls::kH);
+    if (e == hipSuccess) e = g_scratch.dh1.alloc(sizeof(float) * size_t(n) * pls::kH);
+    if (e == hipSuccess) g_scratch.cap = n;
+  }
+  if (e == hipSuccess)
+    e = plk::mlp_backward(P, X, n, pls::kF, pls::kH, pls::kK, h1, h2, out, T, M, G, loss,
+                          g_scratch.dout.as<float>(), g_scratch.dh2.as<float>(),
+                          g_scratch.dh1.as<float>(), pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "MLP backward");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_mlp_adam(float *P, const float *G, float *m, float *v, int step,
+                                        float lr, float beta1, float beta2, float eps) {
+  if (!P || !G || !m || !v || step <= 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  hipError_t e = plk::adam_step(P, G, m, v, int64_t(pls::kNumParams), step, lr, beta1, beta2, eps,
+                                pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "Adam step");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,357 @@
+"""ctypes view of libplsparse_amd.so — the C-ABI declared in include/perflibs_sparse.h
+and include/perflibs_sparse_amd.h.
+
+This module only marshals arguments: every operation runs inside the library
+(C++ host code and gfx950 HIP kernels).  Importing it fails loudly when the
+library has not been built; there is no fallback.
+
+Vectors may be numpy arrays (host memory: the library stages them through HBM)
+or device pointers given as ints (e.g. ``torch_tensor.data_ptr()``), which the
+library uses in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "lib" / "libplsparse_amd.so"
+
+# ---- enum values of perflibs_sparse.h ----
+STATUS_SUCCESS, STATUS_INPUT_PARAMETER_ERROR, STATUS_EXECUTION_FAILURE = 0, 1, 2
+HINT_STRUCTURE, HINT_SPMV_OPERATION, HINT_SPMM_OPERATION = 50, 60, 61
+HINT_MEMORY, HINT_SPMV_INVOCATIONS, HINT_SPMM_INVOCATIONS = 70, 80, 81
+COL_MAJOR, ROW_MAJOR = 90, 91
+STRUCTURE_UNSTRUCTURED = 101
+NOTRANS, TRANS, CONJTRANS = 300, 310, 320
+INVOCATIONS_SINGLE, INVOCATIONS_FEW, INVOCATIONS_MANY = 400, 410, 420
+SCALAR_ONE, SCALAR_ZERO, SCALAR_ANY = 500, 501, 502
+CREATE_NOCOPY = 1
+
+# ---- perflibs_sparse_amd.h ----
+VARIANTS = [
+    "csr_vector", "csr_adaptive",
+    "sell_c32_s0", "sell_c32_s64", "sell_c32_s1024", "sell_c32_s16384",
+    "sell_c64_s0", "sell_c64_s64", "sell_c64_s1024", "sell_c64_s16384",
+    "bsr_b4", "bsr_b8",
+]
+NUM_VARIANTS = len(VARIANTS)
+NUM_FEATURES = 12
+MLP_HIDDEN = 64
+SELECT_LEARNED, SELECT_EXHAUSTIVE, SELECT_HEURISTIC, SELECT_FIXED_CSR = 0, 1, 2, 3
+VALUES_NATIVE, VALUES_BF16 = 0, 2
+
+
+class PerflibsError(RuntimeError):
+    def __init__(self, fn: str, status: int):
+        super().__init__(f"{fn} returned status {status}: {lib().perflibs_amd_last_error().decode()}")
+        self.status = status
+
+
+_lib = None
+
+_I = C.c_int32
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_D = C.c_double
+_F = C.c_float
+
+_SIGS = {
+    "perflibs_spmat_create_csr_s": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_csr_d": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_dense_s": (_I, [_PP, C.c_int, _I, _I, _I, _P, _I]),
+    "perflibs_spmat_create_dense_d": (_I, [_PP, C.c_int, _I, _I, _I, _P, _I]),
+    "perflibs_spmat_create_bsr_s": (_I, [_PP, C.c_int, _I, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_bsr_d": (_I, [_PP, C.c_int, _I, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_null": (_P, [_I, _I]),
+    "perflibs_spmat_create_identity": (_P, [_I]),
+    "perflibs_spmat_destroy": (_I, [_P]),
+    "perflibs_spmat_query": (_I, [_P, _P, _P, _P, _P]),
+    "perflibs_spmat_hint": (_I, [_P, C.c_int, C.c_int]),
+    "perflibs_spmv_optimize": (_I, [_P]),
+    "perflibs_spmm_optimize": (_I, [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P]),
+    "perflibs_spmv_exec_s": (_I, [C.c_int, _F, _P, _P, _F, _P]),
+    "perflibs_spmv_exec_d": (_I, [C.c_int, _D, _P, _P, _D, _P]),
+    "perflibs_spmm_exec_s": (_I, [C.c_int, C.c_int, _F, _P, _P, _F, _P]),
+    "perflibs_spmm_exec_d": (_I, [C.c_int, C.c_int, _D, _P, _P, _D, _P]),
+    "perflibs_spmat_update_s": (_I, [_P, _I, _P, _P, _P]),
+    "perflibs_spmat_update_d": (_I, [_P, _I, _P, _P, _P]),
+    "perflibs_spmat_export_csr_s": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_csr_d": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_dense_s": (_I, [_P, C.c_int, _P, _P, _PP]),
+    "perflibs_spmat_export_dense_d": (_I, [_P, C.c_int, _P, _P, _PP]),
+    "perflibs_spmat_print_err": (None, [_P]),
+    "perflibs_amd_set_stream": (_I, [_P]),
+    "perflibs_amd_get_stream": (_P, []),
+    "perflibs_amd_spmv_exec_async_d": (_I, [C.c_int, _D, _P, _P, _D, _P]),
+    "perflibs_amd_spmv_exec_async_s": (_I, [C.c_int, _F, _P, _P, _F, _P]),
+    "perflibs_amd_spmm_exec_async_d": (_I, [_D, _P, _I, _P, _I, _D, _P, _I]),
+    "perflibs_amd_spmm_exec_async_s": (_I, [_F, _P, _I, _P, _I, _F, _P, _I]),
+    "perflibs_amd_set_select_mode": (_I, [_P, C.c_int]),
+    "perflibs_amd_force_variant": (_I, [_P, C.c_int]),
+    "perflibs_amd_get_variant": (_I, [_P, _P]),
+    "perflibs_amd_set_precision": (_I, [_P, C.c_int]),
+    "perflibs_amd_force_sell": (_I, [_P, _I, _I]),
+    "perflibs_amd_features": (_I, [_P, _P]),
+    "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
+    "perflibs_amd_mlp_num_params": (C.c_size_t, []),
+    "perflibs_amd_set_selector_model": (_I, [_P, _P, _P]),
+    "perflibs_amd_clear_selector_model": (_I, []),
+    "perflibs_amd_mlp_forward": (_I, [_P, _P, C.c_int, _P, _P, _P]),
+    "perflibs_amd_mlp_backward": (_I, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_mlp_adam": (_I, [_P, _P, _P, _P, C.c_int, _F, _F, _F, _F]),
+    "perflibs_amd_sell_info": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_sell_export": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_last_error": (C.c_char_p, []),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libplsparse_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python {PKG / 'build.py'}`")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(fn: str, status: int) -> None:
+    if status != STATUS_SUCCESS:
+        raise PerflibsError(fn, status)
+
+
+def _ptr(a) -> int | None:
+    """Host numpy array -> its address; int -> a device address as given."""
+    if a is None:
+        return None
+    if isinstance(a, (int, np.integer)):
+        return int(a)
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):  # torch tensor
+        return int(a.data_ptr())
+    raise TypeError(f"unsupported buffer {type(a)}")
+
+
+class SpMat:
+    """Owning wrapper of a perflibs_spmat_t handle."""
+
+    def __init__(self, handle: int, dtype, keep=()):
+        self.h = handle
+        self.dtype = np.dtype(dtype)
+        self._keep = keep  # NOCOPY arrays must outlive the handle
+
+    # ---- creation ----
+    @staticmethod
+    def _sfx(dtype) -> str:
+        dt = np.dtype(dtype)
+        if dt == np.float64:
+            return "d"
+        if dt == np.float32:
+            return "s"
+        raise TypeError(f"unsupported dtype {dt}")
+
+    @classmethod
+    def csr(cls, m, n, row_ptr, col_indx, vals, flags=0, raise_on_error=True):
+        vals = np.ascontiguousarray(vals)
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_indx, dtype=np.int32)
+        h = C.c_void_p()
+        fn = f"perflibs_spmat_create_csr_{cls._sfx(vals.dtype)}"
+        st = getattr(lib(), fn)(C.byref(h), m, n, _ptr(rp), _ptr(ci), _ptr(vals), flags)
+        obj = cls(h.value, vals.dtype, keep=(rp, ci, vals))
+        if raise_on_error:
+            check(fn, st)
+            return obj
+        return obj, st
+
+    @classmethod
+    def bsr(cls, layout, m, n, b, row_ptr, col_indx, vals, flags=0, raise_on_error=True):
+        vals = np.ascontiguousarray(vals)
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_indx, dtype=np.int32)
+        h = C.c_void_p()
+        fn = f"perflibs_spmat_create_bsr_{cls._sfx(vals.dtype)}"
+        st = getattr(lib(), fn)(C.byref(h), layout, m, n, b, _ptr(rp), _ptr(ci), _ptr(vals), flags)
+        obj = cls(h.value, vals.dtype, keep=(rp, ci, vals))
+        if raise_on_error:
+            check(fn, st)
+            return obj
+        return obj, st
+
+    @classmethod
+    def dense(cls, layout, m, n, lda, vals, flags=0, raise_on_error=True):
+        vals = np.ascontiguousarray(vals)
+        h = C.c_void_p()
+        fn = f"perflibs_spmat_create_dense_{cls._sfx(vals.dtype)}"
+        st = getattr(lib(), fn)(C.byref(h), layout, m, n, lda, _ptr(vals), flags)
+        obj = cls(h.value, vals.dtype, keep=(vals,))
+        if raise_on_error:
+            check(fn, st)
+            return obj
+        return obj, st
+
+    @classmethod
+    def null(cls, m, n, dtype=np.float64):
+        return cls(lib().perflibs_spmat_create_null(m, n), dtype)
+
+    @classmethod
+    def identity(cls, n, dtype=np.float64):
+        return cls(lib().perflibs_spmat_create_identity(n), dtype)
+
+    def destroy(self):
+        if self.h:
+            lib().perflibs_spmat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def query(self):
+        b, m, n, nnz = (C.c_int32() for _ in range(4))
+        check("query", lib().perflibs_spmat_query(self.h, C.byref(b), C.byref(m), C.byref(n), C.byref(nnz)))
+        return b.value, m.value, n.value, nnz.value
+
+    def hint(self, kind, value) -> int:
+        return lib().perflibs_spmat_hint(self.h, kind, value)
+
+    def optimize(self):
+        check("perflibs_spmv_optimize", lib().perflibs_spmv_optimize(self.h))
+
+    def force_variant(self, v: int):
+        check("force_variant", lib().perflibs_amd_force_variant(self.h, v))
+
+    def force_sell(self, Cc: int, sigma: int):
+        check("force_sell", lib().perflibs_amd_force_sell(self.h, Cc, sigma))
+
+    def select_mode(self, mode: int):
+        check("set_select_mode", lib().perflibs_amd_set_select_mode(self.h, mode))
+
+    def set_precision(self, p: int):
+        check("set_precision", lib().perflibs_amd_set_precision(self.h, p))
+
+    def variant(self) -> int:
+        v = C.c_int()
+        check("get_variant", lib().perflibs_amd_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    # ---- execution ----
+    def spmv(self, x, y, alpha=1.0, beta=0.0, trans=NOTRANS, raise_on_error=True) -> int:
+        fn = f"perflibs_spmv_exec_{self._sfx(self.dtype)}"
+        st = getattr(lib(), fn)(trans, alpha, self.h, _ptr(x), beta, _ptr(y))
+        if raise_on_error:
+            check(fn, st)
+        return st
+
+    def spmv_async(self, x_dev: int, y_dev: int, alpha=1.0, beta=0.0, trans=NOTRANS):
+        fn = f"perflibs_amd_spmv_exec_async_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(trans, alpha, self.h, _ptr(x_dev), beta, _ptr(y_dev)))
+
+    def spmm_async(self, k, B_dev, ldb, C_dev, ldc, alpha=1.0, beta=0.0):
+        fn = f"perflibs_amd_spmm_exec_async_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(alpha, self.h, k, _ptr(B_dev), ldb, beta, _ptr(C_dev), ldc))
+
+    def update(self, rows, cols, vals, raise_on_error=True) -> int:
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=self.dtype)
+        fn = f"perflibs_spmat_update_{self._sfx(self.dtype)}"
+        st = getattr(lib(), fn)(self.h, len(r), _ptr(r), _ptr(c), _ptr(v))
+        if raise_on_error:
+            check(fn, st)
+        return st
+
+    # ---- export ----
+    def export_csr(self, index_base=0):
+        m, n = C.c_int32(), C.c_int32()
+        rp, ci, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        fn = f"perflibs_spmat_export_csr_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(self.h, index_base, C.byref(m), C.byref(n), C.byref(rp), C.byref(ci), C.byref(v)))
+        libc = C.CDLL(None)
+        mm = m.value
+        rpa = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int32)), (mm + 1,)).copy()
+        nnz = int(rpa[-1] - rpa[0])
+        cia = np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy()
+        ctype = C.c_double if self.dtype == np.float64 else C.c_float
+        va = np.ctypeslib.as_array(C.cast(v, C.POINTER(ctype)), (max(nnz, 1),))[:nnz].copy()
+        for p in (rp, ci, v):
+            libc.free(p)
+        return mm, n.value, rpa, cia, va
+
+    def export_dense(self, layout=ROW_MAJOR):
+        m, n = C.c_int32(), C.c_int32()
+        v = C.c_void_p()
+        fn = f"perflibs_spmat_export_dense_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(self.h, layout, C.byref(m), C.byref(n), C.byref(v)))
+        ctype = C.c_double if self.dtype == np.float64 else C.c_float
+        size = m.value * n.value
+        arr = np.ctypeslib.as_array(C.cast(v, C.POINTER(ctype)), (max(size, 1),))[:size].copy()
+        C.CDLL(None).free(v)
+        if layout == ROW_MAJOR:
+            return arr.reshape(m.value, n.value)
+        return arr.reshape(n.value, m.value).T
+
+    # ---- introspection ----
+    def features(self) -> np.ndarray:
+        f = np.zeros(NUM_FEATURES, dtype=np.float32)
+        check("features", lib().perflibs_amd_features(self.h, _ptr(f)))
+        return f
+
+    def time_variants(self, warmup=3, reps=10) -> np.ndarray:
+        ms = np.zeros(NUM_VARIANTS, dtype=np.float64)
+        check("time_variants", lib().perflibs_amd_time_variants(self.h, warmup, reps, _ptr(ms)))
+        return ms
+
+    def sell_layout(self) -> dict:
+        Cc, sig, cb, nf, nt = (C.c_int32() for _ in range(5))
+        nC, padded = C.c_int64(), C.c_int64()
+        check("sell_info", lib().perflibs_amd_sell_info(
+            self.h, C.byref(Cc), C.byref(sig), C.byref(nC), C.byref(padded), C.byref(cb), C.byref(nf), C.byref(nt)))
+        out = dict(C=Cc.value, sigma=sig.value, nC=nC.value, padded=padded.value,
+                   col_indx_bytes=cb.value, n_full_chunks=nf.value, ntidyup=nt.value)
+        ncs = nC.value
+        cs = np.zeros(ncs, np.int64)
+        cl = np.zeros(ncs, np.int64)
+        perm = np.zeros(ncs * Cc.value, np.int64)
+        cmin = np.zeros(ncs, np.int64)
+        offs = np.zeros(max(padded.value, 1), np.int64)
+        vals = np.zeros(max(padded.value, 1), np.float64)
+        check("sell_export", lib().perflibs_amd_sell_export(
+            self.h, _ptr(cs), _ptr(cl), _ptr(perm), _ptr(cmin), _ptr(offs), _ptr(vals)))
+        out.update(cs=cs, cl=cl, row_permd2in=perm, col_indx_min=cmin,
+                   offsets=offs[: padded.value], vals=vals[: padded.value])
+        return out
+
+
+def set_stream(stream_ptr: int) -> None:
+    check("set_stream", lib().perflibs_amd_set_stream(stream_ptr))
+
+
+def spmm_exec(transA, transB, alpha, A: SpMat, B: SpMat, beta, Cm: SpMat, raise_on_error=True) -> int:
+    fn = f"perflibs_spmm_exec_{SpMat._sfx(A.dtype)}"
+    st = getattr(lib(), fn)(transA, transB, alpha, A.h, B.h, beta, Cm.h)
+    if raise_on_error:
+        check(fn, st)
+    return st
+
+
+def spmm_optimize(transA, transB, alpha_hint, A: SpMat, B: SpMat, beta_hint, Cm: SpMat,
+                  raise_on_error=True) -> int:
+    st = lib().perflibs_spmm_optimize(transA, transB, alpha_hint, A.h, B.h, beta_hint, Cm.h)
+    if raise_on_error:
+        check("perflibs_spmm_optimize", st)
+    return st

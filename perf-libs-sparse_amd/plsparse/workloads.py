@@ -1,0 +1,182 @@
+"""Seeded synthetic inputs of the benchmark configurations (numpy arrays), built
+by libplsparse_workloads.so (workloads/gen.cpp).
+
+Sizes and distributions:
+  laplace2d(316)        5-point Laplacian, n = 99,856, nnz = 498,016 (diag 4, off -1)
+  powerlaw(4_194_304)   lognormal(2.5, 1.0) row lengths clipped to [1, 20000],
+                        uniform distinct sorted columns, N(0,1) values
+  stencil27(128)        27-point operator on 128^3, nnz = 55,742,968 (diag 26, off -1)
+  block_stencil(...)    27-point pattern on a block grid, dense N(0,1) b x b blocks
+  banded(...)           banded random rows (the batched configuration)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "lib" / "libplsparse_workloads.so"
+_lib = None
+
+_P = C.c_void_p
+_I = C.c_int32
+_U = C.c_uint64
+_D = C.c_double
+_L = C.c_int64
+
+_SIGS = {
+    "pls_gen_uniform": (None, [_L, _U, _D, _D, _P]),
+    "pls_gen_normal": (None, [_L, _U, _P]),
+    "pls_gen_stencil2d": (_L, [_I, _I, _I, _I, _P, _P, _P]),
+    "pls_gen_stencil3d": (_L, [_I, _I, _I, _I, _I, _P, _P, _P]),
+    "pls_gen_powerlaw_rows": (_L, [_I, _I, _D, _D, _I, _I, _U, _P]),
+    "pls_gen_powerlaw_fill": (None, [_I, _I, _P, _U, _P, _P]),
+    "pls_gen_banded_rows": (_L, [_I, _I, _I, _I, _I, _U, _P]),
+    "pls_gen_banded_fill": (None, [_I, _I, _I, _P, _U, _P, _P]),
+    "pls_gen_random_rows": (_L, [_I, _I, _I, _I, _U, _P]),
+    "pls_gen_random_fill": (None, [_I, _I, _P, _U, _P, _P]),
+    "pls_gen_block_stencil": (_L, [_I, _I, _I, _I, _U, _P, _P, _P]),
+    "pls_gen_block_random_rows": (_L, [_I, _I, _I, _I, _P]),
+    "pls_gen_block_random_fill": (None, [_I, _I, _I, _I, _P, _U, _P, _P]),
+    "pls_gen_threads": (C.c_int, []),
+}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python {PKG / 'build.py'}`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def uniform(n: int, seed: int, lo=-1.0, hi=1.0) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib().pls_gen_uniform(n, seed, lo, hi, _p(out))
+    return out
+
+
+def normal(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib().pls_gen_normal(n, seed, _p(out))
+    return out
+
+
+def _two_phase(m, rows_fn, fill_fn):
+    rp = np.empty(m + 1, np.int32)
+    nnz = rows_fn(rp)
+    col = np.empty(max(nnz, 1), np.int32)
+    val = np.empty(max(nnz, 1), np.float64)
+    fill_fn(rp, col, val)
+    return rp, col[:nnz], val[:nnz]
+
+
+def stencil2d(nx: int, ny: int | None = None, r: int = 1, star: bool = True):
+    ny = nx if ny is None else ny
+    m = nx * ny
+    return _two_phase(
+        m,
+        lambda rp: lib().pls_gen_stencil2d(nx, ny, r, int(star), _p(rp), None, None),
+        lambda rp, c, v: lib().pls_gen_stencil2d(nx, ny, r, int(star), _p(rp), _p(c), _p(v)),
+    )
+
+
+def laplace2d(nx: int):
+    """5-point Laplacian on nx x nx (diagonal 4, off-diagonals -1)."""
+    return stencil2d(nx, nx, 1, True)
+
+
+def stencil3d(nx: int, ny: int | None = None, nz: int | None = None, r: int = 1, star: bool = False):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    m = nx * ny * nz
+    return _two_phase(
+        m,
+        lambda rp: lib().pls_gen_stencil3d(nx, ny, nz, r, int(star), _p(rp), None, None),
+        lambda rp, c, v: lib().pls_gen_stencil3d(nx, ny, nz, r, int(star), _p(rp), _p(c), _p(v)),
+    )
+
+
+def stencil27(n: int):
+    """27-point operator on n^3 (diagonal 26, neighbours -1)."""
+    return stencil3d(n, n, n, 1, False)
+
+
+def powerlaw(n: int, seed: int, mu=2.5, sigma=1.0, lo=1, hi=20000, m: int | None = None):
+    m = n if m is None else m
+    return _two_phase(
+        m,
+        lambda rp: lib().pls_gen_powerlaw_rows(m, n, mu, sigma, lo, hi, seed, _p(rp)),
+        lambda rp, c, v: lib().pls_gen_powerlaw_fill(m, n, _p(rp), seed, _p(c), _p(v)),
+    )
+
+
+def banded(m: int, bw: int, lo: int, hi: int, seed: int, n: int | None = None):
+    n = m if n is None else n
+    return _two_phase(
+        m,
+        lambda rp: lib().pls_gen_banded_rows(m, n, bw, lo, hi, seed, _p(rp)),
+        lambda rp, c, v: lib().pls_gen_banded_fill(m, n, bw, _p(rp), seed, _p(c), _p(v)),
+    )
+
+
+def random_rows(m: int, n: int, lo: int, hi: int, seed: int):
+    return _two_phase(
+        m,
+        lambda rp: lib().pls_gen_random_rows(m, n, lo, hi, seed, _p(rp)),
+        lambda rp, c, v: lib().pls_gen_random_fill(m, n, _p(rp), seed, _p(c), _p(v)),
+    )
+
+
+def block_stencil(bx: int, by: int, bz: int, b: int, seed: int):
+    """BSR arrays (row_ptr, col, vals[nnzb*b*b], row-major blocks)."""
+    mb = bx * by * bz
+    brp = np.empty(mb + 1, np.int32)
+    nnzb = lib().pls_gen_block_stencil(bx, by, bz, b, seed, _p(brp), None, None)
+    bci = np.empty(max(nnzb, 1), np.int32)
+    bval = np.empty(max(nnzb * b * b, 1), np.float64)
+    lib().pls_gen_block_stencil(bx, by, bz, b, seed, _p(brp), _p(bci), _p(bval))
+    return brp, bci[:nnzb], bval[: nnzb * b * b]
+
+
+def block_random(mb: int, b: int, k: int, w: int, seed: int, nb: int | None = None):
+    nb = mb if nb is None else nb
+    brp = np.empty(mb + 1, np.int32)
+    nnzb = lib().pls_gen_block_random_rows(mb, nb, k, w, _p(brp))
+    bci = np.empty(max(nnzb, 1), np.int32)
+    bval = np.empty(max(nnzb * b * b, 1), np.float64)
+    lib().pls_gen_block_random_fill(mb, nb, b, w, _p(brp), seed, _p(bci), _p(bval))
+    return brp, bci[:nnzb], bval[: nnzb * b * b]
+
+
+def bsr_to_csr_rowmajor(brp, bci, bval, b: int):
+    """Expand row-major BSR (0-based) to CSR keeping every stored value."""
+    mb = len(brp) - 1
+    bb = b * b
+    counts = np.diff(brp).astype(np.int64) * b
+    rp = np.zeros(mb * b + 1, np.int64)
+    rp[1:] = np.cumsum(np.repeat(counts, b))
+    nnz = int(rp[-1])
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    blocks = bval.reshape(-1, b, b)
+    for rb in range(mb):
+        k0, k1 = brp[rb], brp[rb + 1]
+        cols = (bci[k0:k1, None] * b + np.arange(b)[None, :]).reshape(-1)
+        for i in range(b):
+            r = rb * b + i
+            col[rp[r]:rp[r + 1]] = cols
+            val[rp[r]:rp[r + 1]] = blocks[k0:k1, i, :].reshape(-1)
+    return rp.astype(np.int32), col, val

@@ -1,0 +1,159 @@
+"""The C-ABI library loads, exports every declared entry point, and its host
+logic (parameter checks, hints, query, export, update) follows the reference.
+None of these tests launches a kernel."""
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(perflibs_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(pls):
+    lib = pls.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_csr_create_parameter_codes(pls):
+    A, st = pls.SpMat.csr(-1, 3, [0, 0], [0], [1.0], raise_on_error=False)
+    assert st == 1
+    A, st = pls.SpMat.csr(2, -3, [0, 0, 0], [0], [1.0], raise_on_error=False)
+    assert st == 1
+    A, st = pls.SpMat.csr(2, 3, [2, 2, 2], [0], [1.0], raise_on_error=False)
+    assert st == 1
+
+
+def test_bsr_create_codes_match_oracle(pls, orc):
+    cases = [(91, 4, 4, 2, [0, 1, 2]), (92, 4, 4, 2, [0, 0, 0]), (91, -4, 4, 2, [0, 0, 0]),
+             (91, 4, -4, 2, [0, 0, 0]), (91, 4, 4, 3, [0, 0, 0]), (91, 4, 4, 2, [2, 2, 2])]
+    for layout, m, n, b, rp in cases:
+        _, st = pls.SpMat.bsr(layout, m, n, b, rp, [0, 1], np.ones(8), raise_on_error=False)
+        code = orc.bsr_check(layout, m, n, b, rp)
+        assert (st == 0) == (code == 0)
+        if code:
+            assert st == 1
+
+
+def test_dense_create_codes(pls):
+    ok = np.ones(6)
+    assert pls.SpMat.dense(pls.ROW_MAJOR, 2, 3, 3, ok, raise_on_error=False)[1] == 0
+    assert pls.SpMat.dense(95, 2, 3, 3, ok, raise_on_error=False)[1] == 1
+    assert pls.SpMat.dense(pls.ROW_MAJOR, 2, 3, 2, ok, raise_on_error=False)[1] == 1  # lda < n
+    assert pls.SpMat.dense(pls.COL_MAJOR, 2, 3, 1, ok, raise_on_error=False)[1] == 1  # lda < m
+
+
+def test_query_and_index_base(pls):
+    A = pls.SpMat.csr(3, 4, [1, 2, 4, 5], [1, 2, 4, 3], [1.0, 2.0, 3.0, 4.0])
+    assert A.query() == (1, 3, 4, 4)
+    B = pls.SpMat.bsr(pls.ROW_MAJOR, 4, 4, 2, [0, 1, 2], [0, 1], np.ones(8))
+    assert B.query() == (0, 4, 4, 8)
+
+
+def test_hint_validation(pls):
+    A = pls.SpMat.csr(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    assert A.hint(pls.HINT_SPMV_OPERATION, pls.TRANS) == 0
+    assert A.hint(pls.HINT_SPMV_OPERATION, pls.INVOCATIONS_MANY) == 1
+    assert A.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_FEW) == 0
+    assert A.hint(pls.HINT_STRUCTURE, pls.STRUCTURE_UNSTRUCTURED) == 0
+    assert A.hint(pls.HINT_STRUCTURE, 180) == 0  # HPCG
+    assert A.hint(pls.HINT_STRUCTURE, 999) == 1
+    assert A.hint(pls.HINT_MEMORY, 200) == 0
+    assert A.hint(12345, 300) == 1
+
+
+def test_invalid_transpose_is_parameter_error(pls):
+    A = pls.SpMat.csr(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    assert A.spmv(np.ones(2), np.zeros(2), trans=999, raise_on_error=False) == 1
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_export_csr_roundtrip(pls, base):
+    rp = np.array([0, 2, 2, 5]) + base
+    ci = np.array([0, 3, 1, 2, 3]) + base
+    v = np.array([1.0, 0.0, 3.0, 4.0, 5.0])
+    A = pls.SpMat.csr(3, 4, rp, ci, v)
+    for out_base in (0, 1):
+        m, n, orp, oci, ov = A.export_csr(out_base)
+        assert (m, n) == (3, 4)
+        assert np.array_equal(orp, rp - base + out_base)
+        assert np.array_equal(oci, ci - base + out_base)
+        assert np.array_equal(ov, v)  # explicit zeros survive on a CSR object
+
+
+def test_export_bsr_as_csr_matches_oracle(pls, orc):
+    rng = np.random.default_rng(2)
+    vals = rng.standard_normal(3 * 9)
+    vals[[1, 5, 13]] = 0.0
+    for layout in (pls.COL_MAJOR, pls.ROW_MAJOR):
+        A = pls.SpMat.bsr(layout, 6, 6, 3, [0, 2, 2, 3], [0, 1, 2], vals)
+        _, _, rp, ci, v = A.export_csr(0)
+        erp, eci, ev = orc.bsr2csr(layout, 6, 6, 3, [0, 2, 2, 3], [0, 1, 2], vals)
+        assert np.array_equal(rp, erp) and np.array_equal(ci, eci) and np.array_equal(v, ev)
+
+
+def test_export_dense_of_sparse_and_dense(pls):
+    A = pls.SpMat.csr(2, 3, [0, 2, 3], [0, 2, 1], [1.0, 2.0, 3.0])
+    D = np.array([[1.0, 0.0, 2.0], [0.0, 3.0, 0.0]])
+    assert np.array_equal(A.export_dense(pls.ROW_MAJOR), D)
+    assert np.array_equal(A.export_dense(pls.COL_MAJOR), D)
+    vals = np.arange(12, dtype=np.float64)
+    Bm = pls.SpMat.dense(pls.COL_MAJOR, 3, 2, 6, vals)  # lda 6 > m: padded input
+    exp = np.stack([vals[0:3], vals[6:9]], axis=1)
+    assert np.array_equal(Bm.export_dense(pls.ROW_MAJOR), exp)
+    N = pls.SpMat.null(2, 3)
+    assert np.array_equal(N.export_dense(), np.zeros((2, 3)))
+    I = pls.SpMat.identity(3)
+    assert np.array_equal(I.export_dense(), np.eye(3))
+
+
+def test_update_semantics(pls):
+    A = pls.SpMat.csr(3, 3, [1, 2, 4, 5], [1, 1, 3, 2], [1.0, 2.0, 3.0, 4.0])
+    assert A.update([1], [1], [9.0], raise_on_error=False) == 0
+    assert A.update([2], [3], [7.0], raise_on_error=False) == 0
+    _, _, _, _, v = A.export_csr(1)
+    assert v.tolist() == [9.0, 2.0, 7.0, 4.0]
+    # not an existing non-zero -> parameter error
+    assert A.update([1, 3], [1, 1], [5.0, 5.0], raise_on_error=False) == 1
+    assert A.update([4], [1], [5.0], raise_on_error=False) == 1
+    assert pls.lib().perflibs_spmat_update_d(A.h, -1, None, None, None) == 1
+    assert pls.lib().perflibs_spmat_update_d(A.h, 0, None, None, None) == 0
+
+
+def test_update_bsr(pls):
+    A = pls.SpMat.bsr(pls.ROW_MAJOR, 4, 4, 2, [0, 1, 2], [0, 1], np.arange(8, dtype=np.float64))
+    assert A.update([1, 2], [0, 3], [-1.0, -2.0], raise_on_error=False) == 0
+    D = A.export_dense()
+    assert D[1, 0] == -1.0 and D[2, 3] == -2.0
+    assert A.update([0], [3], [1.0], raise_on_error=False) == 1  # block (0,1) absent
+
+
+def test_spmm_parameter_checks(pls):
+    A = pls.SpMat.csr(2, 3, [0, 1, 2], [0, 1], [1.0, 2.0])
+    B = pls.SpMat.dense(pls.ROW_MAJOR, 3, 6, 6, np.ones(18))
+    Cbad = pls.SpMat.null(3, 6)
+    assert pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cbad, raise_on_error=False) == 1
+    Cs = pls.SpMat.csr(2, 6, [0, 0, 0], [0], [0.0])
+    # a beta hint of zero needs C null or dense (parameter 7, src/matmul.cpp:1290-1306)
+    st = pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ZERO, Cs,
+                           raise_on_error=False)
+    assert st == 1
+    Bf = pls.SpMat.dense(pls.ROW_MAJOR, 3, 6, 6, np.ones(18, np.float32))
+    C = pls.SpMat.null(2, 6)
+    assert pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, Bf, 0.0, C, raise_on_error=False) == 1
+    assert pls.spmm_exec(999, pls.NOTRANS, 1.0, A, B, 0.0, C, raise_on_error=False) == 1

@@ -1,0 +1,138 @@
+"""GPU tests of the format selector: MLP kernels against the numpy oracle,
+training, features, exhaustive timing and the selection modes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+K = 12
+
+
+def dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def test_mlp_forward_backward_match_oracle(pls):
+    import torch
+
+    rng = np.random.default_rng(0)
+    n = 300
+    P = (rng.standard_normal(mlp.num_params(K)) * 0.2).astype(np.float32)
+    X = rng.standard_normal((n, 12)).astype(np.float32)
+    T = rng.standard_normal((n, K)).astype(np.float32)
+    M = (rng.random((n, K)) < 0.8).astype(np.float32)
+    assert pls.lib().perflibs_amd_mlp_num_params() == len(P)
+    Pd, Xd, Td, Md = dev(P), dev(X), dev(T), dev(M)
+    h1 = torch.empty((n, 64), device="cuda")
+    h2 = torch.empty((n, 64), device="cuda")
+    out = torch.empty((n, K), device="cuda")
+    G = torch.empty(len(P), device="cuda")
+    loss = torch.empty(2, device="cuda")
+    L = pls.lib()
+    pls.check("fwd", L.perflibs_amd_mlp_forward(Pd.data_ptr(), Xd.data_ptr(), n, h1.data_ptr(),
+                                                h2.data_ptr(), out.data_ptr()))
+    pls.check("bwd", L.perflibs_amd_mlp_backward(Pd.data_ptr(), Xd.data_ptr(), n, h1.data_ptr(),
+                                                 h2.data_ptr(), out.data_ptr(), Td.data_ptr(),
+                                                 Md.data_ptr(), G.data_ptr(), loss.data_ptr()))
+    torch.cuda.synchronize()
+    _, _, ref_out = mlp.forward(P.astype(np.float64), X, K)
+    ref_loss, ref_g = mlp.backward(P.astype(np.float64), X, T.astype(np.float64), M, K)
+    assert np.max(np.abs(out.cpu().numpy() - ref_out)) <= 1e-4 * max(1.0, np.abs(ref_out).max())
+    assert abs(loss.cpu().numpy()[0] - ref_loss) <= 1e-4 * ref_loss
+    g = G.cpu().numpy()
+    assert np.linalg.norm(g - ref_g) <= 1e-4 * np.linalg.norm(ref_g)
+
+
+def test_mlp_training_fits(pls):
+    import torch
+
+    rng = np.random.default_rng(1)
+    n = 512
+    X = rng.standard_normal((n, 12)).astype(np.float32)
+    Wt = rng.standard_normal((12, K)).astype(np.float32)
+    T = np.tanh(X @ Wt).astype(np.float32)
+    M = np.ones((n, K), np.float32)
+    P = (rng.standard_normal(mlp.num_params(K)) * 0.1).astype(np.float32)
+    Pd, Xd, Td, Md = dev(P), dev(X), dev(T), dev(M)
+    m1 = torch.zeros_like(Pd)
+    v1 = torch.zeros_like(Pd)
+    h1 = torch.empty((n, 64), device="cuda")
+    h2 = torch.empty((n, 64), device="cuda")
+    out = torch.empty((n, K), device="cuda")
+    G = torch.empty_like(Pd)
+    loss = torch.empty(2, device="cuda")
+    L = pls.lib()
+    losses = []
+    for step in range(1, 801):
+        L.perflibs_amd_mlp_forward(Pd.data_ptr(), Xd.data_ptr(), n, h1.data_ptr(), h2.data_ptr(), out.data_ptr())
+        L.perflibs_amd_mlp_backward(Pd.data_ptr(), Xd.data_ptr(), n, h1.data_ptr(), h2.data_ptr(),
+                                    out.data_ptr(), Td.data_ptr(), Md.data_ptr(), G.data_ptr(), loss.data_ptr())
+        L.perflibs_amd_mlp_adam(Pd.data_ptr(), G.data_ptr(), m1.data_ptr(), v1.data_ptr(), step, 3e-3, 0.9, 0.999, 1e-8)
+        if step in (1, 800):
+            torch.cuda.synchronize()
+            losses.append(float(loss.cpu()[0]))
+    assert losses[1] < 0.1 * losses[0]
+
+
+def test_features_of_known_matrices(pls, gen):
+    rp, ci, v = gen.laplace2d(64)
+    A = pls.SpMat.csr(4096, 4096, rp, ci, v)
+    f = A.features()
+    assert abs(f[0] - np.log(4096)) < 1e-5
+    assert abs(np.expm1(f[3]) - rp[-1] / 4096) < 1e-4  # mean row length
+    assert abs(f[5] - np.log1p(5)) < 1e-5  # longest row
+    assert abs(f[7] - 64 / 4096) < 1e-6  # bandwidth / n
+    assert f[11] == 0.0
+    brp, bci, bv = gen.block_stencil(4, 4, 4, 8, 7)
+    B = pls.SpMat.bsr(pls.ROW_MAJOR, 512, 512, 8, brp, bci, bv)
+    fb = B.features()
+    assert abs(fb[10] - 1.0) < 1e-6 and abs(fb[9] - 1.0) < 1e-6  # fully dense blocks
+    E = pls.SpMat.csr(8, 8, [0, 0, 1, 1, 1, 2, 2, 2, 2], [3, 4], np.ones(2))
+    assert abs(E.features()[11] - 6 / 8) < 1e-6
+
+
+def test_time_variants_and_exhaustive_mode(pls, gen):
+    rp, ci, v = gen.powerlaw(1 << 16, 11)
+    n = 1 << 16
+    A = pls.SpMat.csr(n, n, rp, ci, v)
+    ms = A.time_variants(2, 5)
+    assert np.all(ms[:10] > 0) and np.all(ms[10:] > 0)
+    A.select_mode(pls.SELECT_EXHAUSTIVE)
+    A.optimize()
+    assert 0 <= A.variant() < 12
+    ref = np.zeros(n)
+    x = np.linspace(-1, 1, n)
+    A.spmv(x, ref)
+    B = pls.SpMat.csr(n, n, rp, ci, v)
+    B.select_mode(pls.SELECT_FIXED_CSR)
+    B.optimize()
+    assert B.variant() == 0
+    y = np.zeros(n)
+    B.spmv(x, y)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_learned_mode_uses_model(pls, gen):
+    rng = np.random.default_rng(3)
+    P = (rng.standard_normal(mlp.num_params(K)) * 0.1).astype(np.float32)
+    # bias the output layer so variant 4 is predicted fastest
+    b3 = np.zeros(K, np.float32) + 5.0
+    b3[4] = -5.0
+    P[-K:] = b3
+    L = pls.lib()
+    mean = np.zeros(12, np.float32)
+    std = np.ones(12, np.float32)
+    pls.check("set_model", L.perflibs_amd_set_selector_model(P.ctypes.data, mean.ctypes.data, std.ctypes.data))
+    try:
+        rp, ci, v = gen.laplace2d(100)
+        A = pls.SpMat.csr(10000, 10000, rp, ci, v)
+        A.select_mode(pls.SELECT_LEARNED)
+        A.optimize()
+        assert A.variant() == 4
+    finally:
+        L.perflibs_amd_clear_selector_model()

@@ -1,0 +1,155 @@
+"""GPU parity of CSR x dense SpMM through the C-ABI (FP64 normwise 1e-12
+against the compensated oracle; reference SpMM goldens)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def csr_of_dense(D):
+    rp = np.concatenate([[0], np.cumsum((D != 0).sum(axis=1))]).astype(np.int32)
+    ci = np.concatenate([np.nonzero(r)[0] for r in D]).astype(np.int32)
+    v = np.concatenate([r[np.nonzero(r)[0]] for r in D])
+    return rp, ci, v
+
+
+def test_spmm_null_c_golden(pls, golden):
+    g = golden["spmm_null_c"]
+    A = pls.SpMat.csr(4, 4, g["row_ptr"], g["col_indx"], np.array(g["vals"]))
+    B = pls.SpMat.dense(pls.ROW_MAJOR, 4, 4, 4, np.array(g["B_row_major"]))
+    Cm = pls.SpMat.null(4, 4)
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ZERO, Cm)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cm)
+    exp = np.array(g["dense_A"]).reshape(4, 4) @ np.array(g["B_row_major"]).reshape(4, 4)
+    assert np.max(np.abs(Cm.export_dense() - exp)) <= g["tol"]
+
+
+def test_spmm_variants_golden(pls, golden):
+    g = golden["spmm_variants"]
+    DA = np.array(g["dense_A"]).reshape(4, 4)
+    DB = np.array(g["B_row_major"]).reshape(4, 4)
+    A = pls.SpMat.csr(4, 4, g["row_ptr"], g["col_indx"], np.array(g["vals"]))
+    B = pls.SpMat.dense(pls.ROW_MAJOR, 4, 4, 4, DB.reshape(-1))
+    opt = pls.SpMat.null(4, 4)
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ANY, opt)
+    for ta in (pls.NOTRANS, pls.TRANS):
+        for tb in (pls.NOTRANS, pls.TRANS):
+            alpha = g["alpha_notrans"] if ta == pls.NOTRANS else g["alpha_trans"]
+            Cm = pls.SpMat.null(4, 4)
+            pls.spmm_exec(ta, tb, alpha, A, B, 0.0, Cm)
+            opA = DA if ta == pls.NOTRANS else DA.T
+            opB = DB if tb == pls.NOTRANS else DB.T
+            assert np.max(np.abs(Cm.export_dense() - alpha * opA @ opB)) <= g["tol"]
+    c0 = np.array(g["c_init_row_major"]).reshape(4, 4)
+    Cd = pls.SpMat.dense(pls.ROW_MAJOR, 4, 4, 4, c0.reshape(-1))
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, g["alpha_beta"], A, B, g["beta"], Cd)
+    exp = g["alpha_beta"] * DA @ DB + g["beta"] * c0
+    assert np.max(np.abs(Cd.export_dense() - exp)) <= g["tol"]
+    Bc = pls.SpMat.dense(pls.COL_MAJOR, 4, 4, 4, DB.T.reshape(-1))
+    Cc = pls.SpMat.dense(pls.COL_MAJOR, 4, 4, 4, c0.T.reshape(-1))
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, Bc, pls.SCALAR_ANY, Cc)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, g["alpha_beta"], A, Bc, g["beta"], Cc)
+    assert np.max(np.abs(Cc.export_dense() - exp)) <= g["tol"]
+
+
+@pytest.mark.parametrize("k", [1, 3, 4, 5, 8, 17, 32, 100, 128, 300])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("beta", [0.0, 1.0, -0.5])
+def test_spmm_parity(pls, orc, gen, k, dtype, beta):
+    rng = np.random.default_rng(k)
+    rp, ci, v = gen.stencil3d(16, 12, 10, 1, False)
+    v = v * rng.uniform(0.5, 1.5, len(v))
+    m = 16 * 12 * 10
+    Bv = rng.uniform(-1, 1, (m, k))
+    C0 = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, m, rp, ci, v.astype(dtype))
+    B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.astype(dtype).reshape(-1))
+    Cm = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, C0.astype(dtype).reshape(-1))
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ANY, Cm)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.5, A, B, beta, Cm)
+    if dtype == np.float64:
+        ref = orc.spmm(m, k, rp, ci, v, Bv, 1.5, beta, C0)
+        assert rel_err(Cm.export_dense(), ref) <= TOL64
+    else:
+        ref = orc.spmm(m, k, rp, ci, v.astype(np.float32).astype(np.float64),
+                       Bv.astype(np.float32).astype(np.float64), 1.5, beta,
+                       C0.astype(np.float32).astype(np.float64))
+        assert rel_err(Cm.export_dense(), ref) <= 1e-6
+
+
+def test_spmm_bsr_operand_and_transposes(pls, orc, gen):
+    brp, bci, bv = gen.block_stencil(6, 5, 4, 4, 0x5EED0003)
+    m = 6 * 5 * 4 * 4
+    k = 16
+    rng = np.random.default_rng(3)
+    Bv = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.bsr(pls.ROW_MAJOR, m, m, 4, brp, bci, bv)
+    B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.reshape(-1))
+    Cm = pls.SpMat.null(m, k)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cm)
+    rp, ci, v = orc.bsr2csr(91, m, m, 4, brp, bci, bv)
+    assert rel_err(Cm.export_dense(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
+    # op(A) = A^T, op(B) = B^T via a column-major B' of shape k x m
+    D = np.zeros((m, m))
+    for i in range(m):
+        D[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    Bt = pls.SpMat.dense(pls.ROW_MAJOR, k, m, m, Bv.T.copy().reshape(-1))
+    A2 = pls.SpMat.csr(m, m, rp, ci, v)
+    C2 = pls.SpMat.null(m, k)
+    pls.spmm_exec(pls.TRANS, pls.TRANS, 0.75, A2, Bt, 0.0, C2)
+    assert rel_err(C2.export_dense(), 0.75 * D.T @ Bv) <= TOL64
+
+
+def test_spmm_trivial_cases(pls):
+    A = pls.SpMat.csr(3, 3, [0, 1, 2, 3], [0, 1, 2], np.ones(3))
+    B = pls.SpMat.dense(pls.ROW_MAJOR, 3, 6, 6, np.ones(18))
+    C0 = np.arange(18, dtype=np.float64)
+    Cm = pls.SpMat.dense(pls.ROW_MAJOR, 3, 6, 6, C0)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 0.0, A, B, 2.0, Cm)
+    assert np.array_equal(Cm.export_dense().reshape(-1), 2 * C0)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 0.0, A, B, 0.0, Cm)  # C becomes null
+    assert np.array_equal(Cm.export_dense(), np.zeros((3, 6)))
+    assert Cm.query()[3] == 0
+
+
+def test_spmm_async_device(pls, orc, gen):
+    import torch
+
+    rp, ci, v = gen.stencil27(24)
+    m = 24**3
+    k = 32
+    rng = np.random.default_rng(4)
+    Bv = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.zeros((m, k), dtype=torch.float64, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
+
+
+def test_stencil27_full_k8(pls, orc, gen):
+    """The 128^3 operator at its real size with k = 8 right-hand sides."""
+    rp, ci, v = gen.stencil27(128)
+    m = 128**3
+    assert rp[-1] == 55_742_968
+    k = 8
+    Bv = gen.uniform(m * k, 0x5EED0003).reshape(m, k)
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.reshape(-1))
+    Cm = pls.SpMat.null(m, k)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cm)
+    got = Cm.export_dense()
+    ref = orc.spmm(m, k, rp, ci, v, Bv)
+    assert rel_err(got, ref) <= TOL64
+    # closed form: with B = 1 interior rows vanish (26 - 26 neighbours)
+    ones = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, np.ones(m * k))
+    C1 = pls.SpMat.null(m, k)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, ones, 0.0, C1)
+    g = C1.export_dense()[:, 0].reshape(128, 128, 128)
+    assert np.all(g[1:-1, 1:-1, 1:-1] == 0.0) and np.all(g[0] > 0)
