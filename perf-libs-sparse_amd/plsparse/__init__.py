@@ -114,6 +114,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python {PKG / 'build.py'}`")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (same
+        # SONAME, different file).  Load it first so this library binds to it
+        # instead of pulling /opt/rocm's copy in beside it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)

@@ -101,7 +101,9 @@ def test_time_variants_and_exhaustive_mode(pls, gen):
     n = 1 << 16
     A = pls.SpMat.csr(n, n, rp, ci, v)
     ms = A.time_variants(2, 5)
-    assert np.all(ms[:10] > 0) and np.all(ms[10:] > 0)
+    assert np.all(ms[:10] > 0)
+    # random columns: 4x4 / 8x8 blocks would be ~1/16 full, so BSR is not a candidate
+    assert np.all(ms[10:] < 0)
     A.select_mode(pls.SELECT_EXHAUSTIVE)
     A.optimize()
     assert 0 <= A.variant() < 12
