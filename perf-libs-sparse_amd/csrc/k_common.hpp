@@ -61,6 +61,27 @@ template <int W, class T> __device__ __forceinline__ T group_sum_c(T v) {
   return v;
 }
 
+// Streams non-zeros [j0 + k*stride] for k < N with every index clamped to
+// `last`, so all N column-index and value loads issue before any x gather and
+// the N gathers issue together (no branch splits the memory pipeline).
+template <int N, class V, class X>
+__device__ __forceinline__ void gather_products(const int32_t *__restrict__ ci,
+                                                const V *__restrict__ val,
+                                                const X *__restrict__ x, int32_t j0,
+                                                int32_t stride, int32_t last,
+                                                acc_t<X> (&prod)[N]) {
+  int32_t c[N];
+  acc_t<X> a[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int32_t j = min(j0 + k * stride, last);
+    c[k] = ldi(ci, j);
+    a[k] = acc_t<X>(ldv(val, j));
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) prod[k] = a[k] * acc_t<X>(x[c[k]]);
+}
+
 constexpr int kBlock = 256;  // threads per workgroup (4 waves of 64)
 
 inline unsigned grid_for(int64_t threads) {

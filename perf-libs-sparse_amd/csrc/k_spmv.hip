@@ -25,17 +25,18 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(
   const int64_t t = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
   const int64_t row = t / L;
   const int lane = threadIdx.x & (L - 1);
-  A s0 = 0, s1 = 0;
+  A s = 0;
   if (row < m) {
     const int32_t e = rp[row + 1];
-    int32_t j = rp[row] + lane;
-    for (; j + L < e; j += 2 * L) {
-      s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
-      s1 += A(ldv(val, j + L)) * A(x[ldi(ci, j + L)]);
+    constexpr int U = 4;
+    for (int32_t j = rp[row] + lane; j < e; j += U * L) {
+      A p[U];
+      gather_products<U>(ci, val, x, j, L, e - 1, p);
+#pragma unroll
+      for (int k = 0; k < U; ++k) s += j + k * L < e ? p[k] : A(0);
     }
-    if (j < e) s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
   }
-  A s = group_sum_c<L>(s0 + s1);
+  s = group_sum_c<L>(s);
   if (row < m && lane == 0) {
     A r = alpha * s;
     if (BETA) r += beta * A(y[row]);
@@ -54,7 +55,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(
 // workgroup.  Rows split between threads in the reference (combined there
 // with atomics, src/compressed_sparse_rows.cpp:1146-1203) never occur: every
 // row is finished by exactly one workgroup.
-constexpr int kAdaNnz = 2048;
+constexpr int kAdaVpt = 8;                  // non-zeros staged per thread
+constexpr int kAdaNnz = kBlock * kAdaVpt;   // non-zeros per row block
 
 template <class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_csr_adaptive(
@@ -71,14 +73,16 @@ __global__ __launch_bounds__(kBlock) void k_csr_adaptive(
   const int32_t nnz = e1 - e0, nrows = r1 - r0;
 
   if (nnz > kAdaNnz) {  // one long row (nrows == 1 by construction)
-    A s0 = 0, s1 = 0;
-    int32_t j = e0 + tid;
-    for (; j + kBlock < e1; j += 2 * kBlock) {
-      s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
-      s1 += A(ldv(val, j + kBlock)) * A(x[ldi(ci, j + kBlock)]);
+    constexpr int U = 4;
+    A s = 0;
+    for (int32_t j = e0 + tid; j < e1; j += U * kBlock) {
+      A p[U];
+      gather_products<U>(ci, val, x, j, kBlock, e1 - 1, p);
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (j + k * kBlock < e1) s += p[k];
     }
-    if (j < e1) s0 += A(ldv(val, j)) * A(x[ldi(ci, j)]);
-    A s = group_sum_c<64>(s0 + s1);
+    s = group_sum_c<64>(s);
     if ((tid & 63) == 0) part[tid >> 6] = s;
     __syncthreads();
     if (tid == 0) {
@@ -90,10 +94,14 @@ __global__ __launch_bounds__(kBlock) void k_csr_adaptive(
     return;
   }
 
+  if (nnz > 0) {
+    A p[kAdaVpt];
+    gather_products<kAdaVpt>(ci, val, x, e0 + tid, kBlock, e1 - 1, p);
 #pragma unroll
-  for (int k = 0; k < kAdaNnz / kBlock; ++k) {
-    const int q = tid + k * kBlock;
-    if (q < nnz) prod[q] = A(ldv(val, e0 + q)) * A(x[ldi(ci, e0 + q)]);
+    for (int k = 0; k < kAdaVpt; ++k) {
+      const int q = tid + k * kBlock;
+      if (q < nnz) prod[q] = p[k];
+    }
   }
   __syncthreads();
 
@@ -152,20 +160,25 @@ __global__ __launch_bounds__(kBlock) void k_sell(
   const int64_t base = cs[c] + r;
   const int32_t w = cl[c];
   const X *xm = x + cmin[c];
-  A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  int32_t j = 0;
-  for (; j + 3 < w; j += 4) {
-    const int64_t k = base + int64_t(j) * C;
-    s0 += A(ldv(val, k)) * A(xm[ldo(offs, k)]);
-    s1 += A(ldv(val, k + C)) * A(xm[ldo(offs, k + C)]);
-    s2 += A(ldv(val, k + 2 * C)) * A(xm[ldo(offs, k + 2 * C)]);
-    s3 += A(ldv(val, k + 3 * C)) * A(xm[ldo(offs, k + 3 * C)]);
+  // U slots per step: U offset and U value loads issue back to back, then
+  // the U gathers; the tail step clamps to the last slot and masks it.
+  constexpr int U = 8;
+  A s = 0;
+  for (int32_t j = 0; j < w; j += U) {
+    uint32_t o[U];
+    A a[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t at = base + int64_t(min(j + k, w - 1)) * C;
+      o[k] = ldo(offs, at);
+      a[k] = A(ldv(val, at));
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const A p = a[k] * A(xm[o[k]]);
+      s += j + k < w ? p : A(0);
+    }
   }
-  for (; j < w; ++j) {
-    const int64_t k = base + int64_t(j) * C;
-    s0 += A(ldv(val, k)) * A(xm[ldo(offs, k)]);
-  }
-  const A s = (s0 + s1) + (s2 + s3);
   const int64_t pr = c * C + r;
   const int32_t row = PERM ? perm[pr] : int32_t(pr);
   if (PERM ? row >= 0 : pr < m) {
@@ -199,8 +212,22 @@ __global__ __launch_bounds__(kBlock) void k_bsr_wave(
   if (wave >= mb) return;
   const int32_t k0 = rp[wave], k1 = rp[wave + 1];
   A s = 0;
-  for (int32_t k = k0 + slot; k < k1; k += PER)
-    s += A(ldv(val, int64_t(k) * BB + e)) * A(x[int64_t(ldi(ci, k)) * B + j]);
+  constexpr int U = 4;  // block steps in flight per lane
+  for (int32_t k = k0 + slot; k < k1; k += U * PER) {
+    int32_t c[U];
+    A a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t kk = min(k + u * PER, k1 - 1);
+      c[u] = ldi(ci, kk);
+      a[u] = A(ldv(val, int64_t(kk) * BB + e));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const A p = a[u] * A(x[int64_t(c[u]) * B + j]);
+      s += k + u * PER < k1 ? p : A(0);
+    }
+  }
   // fold j (lanes differing in the j bits) and the block slots
   if (COLMAJ) {
 #pragma unroll
