@@ -40,7 +40,8 @@ enum perflibs_amd_variant {
   PERFLIBS_AMD_SELL_C64_S16384 = 9,
   PERFLIBS_AMD_BSR_B4 = 10,      /* blocked 4x4 (converted from CSR when needed) */
   PERFLIBS_AMD_BSR_B8 = 11,
-  PERFLIBS_AMD_NUM_VARIANTS = 12
+  PERFLIBS_AMD_CSR_PANEL = 12,   /* row tiles swept over column panels of x, LDS row sums */
+  PERFLIBS_AMD_NUM_VARIANTS = 13
 };
 
 /* ---- how perflibs_spmv_optimize picks a variant ---- */
@@ -90,6 +91,18 @@ perflibs_status_t perflibs_amd_set_precision(perflibs_spmat_t A, int precision);
 /* Force SELL-C-sigma with any C in [1, 64] and sigma a multiple of C (or 0). */
 perflibs_status_t perflibs_amd_force_sell(perflibs_spmat_t A, perflibs_int_t C,
                                           perflibs_int_t sigma);
+
+/* Force the column-panel variant with panels of 2^panel_log2 columns
+ * (1 <= panel_log2 <= 24, or 0 for the default), at most wg_per_cu resident
+ * workgroups of 4 wave tiles per CU (1..16, or 0 for the default) and a pacing
+ * window of `window` panels (0: waves never wait; -1: the default). */
+perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t panel_log2,
+                                           perflibs_int_t wg_per_cu, perflibs_int_t window);
+/* Shape of a column-panel plan (for tests and tuning). */
+perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
+                                          perflibs_int_t *npanels, perflibs_int_t *ntiles,
+                                          perflibs_int_t *rows_per_tile_max,
+                                          perflibs_int_t *grid);
 
 /* Matrix features used by the selector (PERFLIBS_AMD_NUM_FEATURES floats),
  * computed on the GPU from the CSR structure. */

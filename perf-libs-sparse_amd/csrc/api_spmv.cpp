@@ -262,6 +262,37 @@ perflibs_status_t perflibs_amd_force_variant(perflibs_spmat_t A, int variant) {
   A->forced_variant = variant;
   A->forced_C = -1;
   A->forced_sigma = -1;
+  A->forced_panel_log2 = -1;
+  A->forced_panel_wpc = -1;
+  A->forced_panel_window = -1;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t panel_log2,
+                                           perflibs_int_t wg_per_cu, perflibs_int_t window) {
+  if (!A || panel_log2 < 0 || panel_log2 > 24 || wg_per_cu < 0 || wg_per_cu > 16 ||
+      window < -1 || window > 64)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  A->forced_panel_window = window;
+  A->forced_variant = PERFLIBS_AMD_CSR_PANEL;
+  A->forced_C = -1;
+  A->forced_sigma = -1;
+  A->forced_panel_log2 = panel_log2 > 0 ? panel_log2 : -1;
+  A->forced_panel_wpc = wg_per_cu > 0 ? wg_per_cu : -1;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
+                                          perflibs_int_t *npanels, perflibs_int_t *ntiles,
+                                          perflibs_int_t *rows_per_tile_max,
+                                          perflibs_int_t *grid) {
+  if (!A || !A->plan.panel) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  const pls::DevPanel &p = *A->plan.panel;
+  if (panel_log2) *panel_log2 = p.pw_log2;
+  if (npanels) *npanels = p.npanels;
+  if (ntiles) *ntiles = p.ntiles;
+  if (rows_per_tile_max) *rows_per_tile_max = p.rcap;
+  if (grid) *grid = p.grid;
   return PERFLIBS_STATUS_SUCCESS;
 }
 

@@ -7,7 +7,10 @@
 //   SELL-C-sigma: chunk offsets int64[nC], widths int32[nC], permutation
 //     int32[nC*C] (-1 = padding row), chunk column minimum int32[nC], column
 //     offsets uint8/16/32[padded], vals[padded] (column-major per chunk);
-//   BSR: block row_ptr int32[mb+1], block col int32[nnzb], vals[nnzb*b*b].
+//   BSR: block row_ptr int32[mb+1], block col int32[nnzb], vals[nnzb*b*b];
+//   column panel: wave-tile first rows int32[ntiles+1], (tile, panel) segment
+//     offsets int64[ntiles*npanels+1], packed (row, column) uint32[nnz],
+//     vals[nnz] (tile-major, then panel-major, rows in order).
 // vals are FP64 for _d handles, FP32 or BF16 for _s handles.
 #include <algorithm>
 #include <cmath>
@@ -19,6 +22,7 @@ namespace pls {
 namespace {
 constexpr int kAdaNnz = 2048;       // must equal plk::kAdaNnz (k_spmv.hip)
 constexpr int kAdaMaxRows = 1024;   // rows per adaptive block
+constexpr int kPanelWindow = 0;     // column-panel pacing window (panels)
 
 struct SellParams {
   int32_t C, sigma;
@@ -126,6 +130,7 @@ bool variant_applicable(const perflibs_spmat_top_t *A, const Operand &o, int v) 
   if (is_sell(v)) return o.m > 0;
   if (v == PERFLIBS_AMD_BSR_B4) return o.m % 4 == 0 && o.n % 4 == 0 && o.m > 0;
   if (v == PERFLIBS_AMD_BSR_B8) return o.m % 8 == 0 && o.n % 8 == 0 && o.m > 0;
+  if (v == PERFLIBS_AMD_CSR_PANEL) return o.m > 0;
   return false;
 }
 
@@ -255,9 +260,81 @@ perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A) {
                     PERFLIBS_SPARSE_OPERATION_NOTRANS, variant);
 }
 
+// Column-panel plan.  Shape: panels of 2^pw columns (1 MiB of x by default),
+// and as many wave tiles as the GPU keeps resident at the largest occupancy
+// (workgroups per CU, at most 8) whose LDS accumulators hold 1.25x the mean
+// rows per tile; tiles beyond the resident count run in further rounds.
+static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
+                                          int variant) {
+  const bool dbl = A->dt == DType::F64;
+  const VStore vs = value_store(A);
+  const size_t abytes = dbl ? 8 : 4;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return hip_fail(A, e, "device query");
+  cus = std::max(cus, 1);
+  const int32_t pw = A->forced_panel_log2 > 0 ? A->forced_panel_log2 : (dbl ? 17 : 18);
+  const int32_t rmax_idx = int32_t(std::min<int64_t>(int64_t(1) << (32 - pw), 1 << 20));
+  constexpr size_t kMaxLds = 64 * 1024;  // per workgroup
+  const int32_t rmax_lds = int32_t(kMaxLds / (4 * abytes));
+  const int32_t rmax = std::min(rmax_idx, rmax_lds);
+  int wpc_hi = A->forced_panel_wpc > 0 ? A->forced_panel_wpc : 4;
+  int wpc = 0;
+  int32_t rcap = 0;
+  for (int w = wpc_hi; w >= 1 && wpc == 0; --w) {
+    const int64_t tiles = int64_t(cus) * w * 4;
+    int64_t rc = (int64_t(o.m) * 5 + tiles * 4 - 1) / (tiles * 4);  // 1.25 * m / tiles
+    rc = std::max<int64_t>(64, (rc + 63) / 64 * 64);
+    if (rc > rmax) continue;
+    if (plk::csr_panel_occupancy(dbl, vs, size_t(4) * rc * abytes) < w &&
+        A->forced_panel_wpc <= 0)
+      continue;
+    wpc = w;
+    rcap = int32_t(rc);
+  }
+  if (wpc == 0) {  // more rows than one round of resident tiles can hold
+    wpc = A->forced_panel_wpc > 0
+              ? A->forced_panel_wpc
+              : std::max(1, plk::csr_panel_occupancy(dbl, vs, size_t(4) * rmax * abytes));
+    rcap = rmax;
+  }
+  PanelHost P;
+  if (!panel_build(o.m, o.n, o.prp, o.pci, o.pvals, A->dt, cus * wpc * 4, rcap, pw, P))
+    return exec_fail(A, "column-panel layout does not fit 32-bit packed indices");
+  auto d = std::make_unique<DevPanel>();
+  d->m = o.m;
+  d->n = o.n;
+  d->nnz = int64_t(o.prp[o.m]) - o.prp[0];
+  d->pw_log2 = pw;
+  d->npanels = P.npanels;
+  d->ntiles = P.ntiles;
+  d->rcap = rcap;
+  d->grid = int32_t(std::min<int64_t>(P.ntiles / 4, int64_t(cus) * wpc));
+  d->vs = vs;
+  d->window = A->forced_panel_window >= 0 ? A->forced_panel_window : kPanelWindow;
+  if (d->npanels > 1 && d->window > 0) {
+    const size_t sb = size_t(8) * d->npanels * 128;
+    e = d->sync.alloc(sb);
+    if (e == hipSuccess) e = hipMemset(d->sync.get(), 0, sb);
+    if (e != hipSuccess) return hip_fail(A, e, "column-panel plan upload");
+  }
+  e = d->tile_row0.upload(P.tile_row0.data(), sizeof(int32_t) * P.tile_row0.size());
+  if (e == hipSuccess) e = d->seg.upload(P.seg.data(), sizeof(int64_t) * P.seg.size());
+  if (e == hipSuccess) e = d->idx.upload(P.idx.data(), sizeof(uint32_t) * P.idx.size());
+  if (e == hipSuccess) e = upload_vals(d->vals, P.vals.data(), int64_t(P.idx.size()), A->dt, vs);
+  if (e != hipSuccess) return hip_fail(A, e, "column-panel plan upload");
+  A->plan.clear();
+  A->plan.panel = std::move(d);
+  A->plan.variant = variant;
+  A->plan.op = op;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op, int variant) {
   if (variant == PERFLIBS_AMD_CSR_VECTOR || variant == PERFLIBS_AMD_CSR_ADAPTIVE)
     return build_csr_plan(A, o, op, variant);
+  if (variant == PERFLIBS_AMD_CSR_PANEL) return build_panel_plan(A, o, op, variant);
   if (is_sell(variant)) {
     const SellParams p = sell_params(variant);
     return build_sell_plan(A, o, op, p.C, p.sigma, variant);
@@ -294,6 +371,8 @@ perflibs_status_t launch_plan(perflibs_spmat_top_t *A, double alpha, const void 
     e = plk::sell_spmv(*A->plan.sell, dbl, alpha, x, beta, y, stream());
   } else if (A->plan.bsr) {
     e = plk::bsr_spmv(*A->plan.bsr, dbl, alpha, x, beta, y, stream());
+  } else if (A->plan.panel) {
+    e = plk::csr_panel(*A->plan.panel, dbl, alpha, x, beta, y, stream());
   } else {
     return exec_fail(A, "no device plan");
   }

@@ -125,6 +125,25 @@ struct DevSell {
   std::vector<int64_t> h_cs, h_cl, h_permd2in, h_in2permd, h_cmin;
 };
 
+// Device column-panel CSR (layout built by panel_build, panel_build.cpp).
+struct DevPanel {
+  int32_t m = 0, n = 0;
+  int64_t nnz = 0;
+  int32_t pw_log2 = 18, npanels = 1, ntiles = 0, rcap = 0;
+  int32_t grid = 0;  // persistent workgroups (4 wave tiles each per round)
+  VStore vs = VStore::F64;
+  DevBuf tile_row0;  // int32 [ntiles+1]
+  DevBuf seg;        // int64 [ntiles*npanels+1], quad-aligned start | padding count
+  DevBuf idx;        // uint32 [padded nnz], (row - tile row0) << pw_log2 | col - panel col0
+  DevBuf vals;       // [padded nnz]
+  // Pacing of the panel sweep: per-XCD, per-panel counters of finished waves
+  // (8 * npanels lines of 128 bytes, zeroed at build), the window in panels
+  // (0: no pacing) and the number of launches so far.
+  DevBuf sync;
+  int32_t window = 0;
+  mutable uint32_t launches = 0;
+};
+
 struct DevBsr {
   int layout = PERFLIBS_ROW_MAJOR;
   int32_t b = 0, mb = 0, nb = 0;
@@ -140,7 +159,8 @@ struct Plan {
   std::unique_ptr<DevCsr> csr;
   std::unique_ptr<DevSell> sell;
   std::unique_ptr<DevBsr> bsr;
-  void clear() { variant = -1; csr.reset(); sell.reset(); bsr.reset(); }
+  std::unique_ptr<DevPanel> panel;
+  void clear() { variant = -1; csr.reset(); sell.reset(); bsr.reset(); panel.reset(); }
 };
 
 }  // namespace pls
@@ -176,6 +196,7 @@ struct perflibs_spmat_top_t {
   int select_mode = -1;   // -1: learned when a model is loaded, else heuristic
   int forced_variant = -1;
   int forced_C = -1, forced_sigma = -1;
+  int forced_panel_log2 = -1, forced_panel_wpc = -1, forced_panel_window = -1;  // column-panel shape overrides
   int precision = PERFLIBS_AMD_VALUES_NATIVE;
 
   pls::DevBuf stage_x, stage_y;  // staging of host x / y
@@ -226,6 +247,20 @@ bool sell_structure(int32_t m, const int32_t *rp, const int32_t *ci, int32_t C,
 void sell_fill(const SellHost &s, int32_t m, const int32_t *rp, const int32_t *ci,
                const void *vals, DType dt, std::vector<uint8_t> &vals_out,
                std::vector<uint8_t> &offs_out);
+
+// ---- column-panel CSR host build (panel_build.cpp) ----
+struct PanelHost {
+  int32_t m = 0, n = 0, pw_log2 = 0, npanels = 0, ntiles = 0;
+  std::vector<int32_t> tile_row0;  // [ntiles+1], ntiles a multiple of 4
+  std::vector<int64_t> seg;        // [ntiles*npanels+1] padded start | padding count
+  std::vector<uint32_t> idx;       // [padded nnz]
+  std::vector<uint8_t> vals;       // [padded nnz] values of the handle dtype
+};
+// Wave tiles of at most rcap rows balanced to about nnz/want_tiles non-zeros,
+// panels of 2^pw_log2 columns; 0-based CSR input.  False if the packed index
+// (rcap << pw_log2) does not fit 32 bits.
+bool panel_build(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci, const void *vals,
+                 DType dt, int32_t want_tiles, int32_t rcap, int32_t pw_log2, PanelHost &P);
 
 // ---- plans (plan.cpp) ----
 // Host CSR operand of op(A) (0-based) used to build any plan.
@@ -279,6 +314,11 @@ hipError_t sell_spmv(const pls::DevSell &A, bool dbl, double alpha, const void *
                      double beta, void *y, hipStream_t s);
 hipError_t bsr_spmv(const pls::DevBsr &A, bool dbl, double alpha, const void *x,
                     double beta, void *y, hipStream_t s);
+hipError_t csr_panel(const pls::DevPanel &A, bool dbl, double alpha, const void *x,
+                     double beta, void *y, hipStream_t s);
+// Resident workgroups per CU of the column-panel kernel with `lds_bytes` of
+// accumulator LDS per workgroup (0 on error).
+int csr_panel_occupancy(bool dbl, pls::VStore vs, size_t lds_bytes);
 // y = beta*y (beta != 0) or y = 0 (beta == 0), n elements.
 hipError_t scale_vec(bool dbl, int64_t n, double beta, void *y, hipStream_t s);
 // y = alpha*x + beta*y.

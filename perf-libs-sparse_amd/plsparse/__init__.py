@@ -35,7 +35,7 @@ VARIANTS = [
     "csr_vector", "csr_adaptive",
     "sell_c32_s0", "sell_c32_s64", "sell_c32_s1024", "sell_c32_s16384",
     "sell_c64_s0", "sell_c64_s64", "sell_c64_s1024", "sell_c64_s16384",
-    "bsr_b4", "bsr_b8",
+    "bsr_b4", "bsr_b8", "csr_panel",
 ]
 NUM_VARIANTS = len(VARIANTS)
 NUM_FEATURES = 12
@@ -94,6 +94,8 @@ _SIGS = {
     "perflibs_amd_get_variant": (_I, [_P, _P]),
     "perflibs_amd_set_precision": (_I, [_P, C.c_int]),
     "perflibs_amd_force_sell": (_I, [_P, _I, _I]),
+    "perflibs_amd_force_panel": (_I, [_P, _I, _I, _I]),
+    "perflibs_amd_panel_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
     "perflibs_amd_mlp_num_params": (C.c_size_t, []),
@@ -244,6 +246,15 @@ class SpMat:
 
     def force_sell(self, Cc: int, sigma: int):
         check("force_sell", lib().perflibs_amd_force_sell(self.h, Cc, sigma))
+
+    def force_panel(self, panel_log2: int = 0, wg_per_cu: int = 0, window: int = -1):
+        check("force_panel", lib().perflibs_amd_force_panel(self.h, panel_log2, wg_per_cu, window))
+
+    def panel_info(self) -> dict:
+        v = [C.c_int32() for _ in range(5)]
+        check("panel_info", lib().perflibs_amd_panel_info(self.h, *[C.byref(a) for a in v]))
+        return dict(zip(("panel_log2", "npanels", "ntiles", "rows_per_tile_max", "grid"),
+                        (a.value for a in v)))
 
     def select_mode(self, mode: int):
         check("set_select_mode", lib().perflibs_amd_set_select_mode(self.h, mode))

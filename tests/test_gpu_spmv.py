@@ -15,7 +15,8 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-ALL_VARIANTS = list(range(12))
+ALL_VARIANTS = list(range(13))
+PANEL = 12
 
 
 def powerlawish(rng, m, n, mean=8.0, empty=0.05, base=0):
@@ -146,7 +147,7 @@ def test_one_based_and_long_rows(pls, orc):
     # a few very long rows (one workgroup each in the adaptive kernel)
     x = rng.uniform(-1, 1, n)
     ref = orc.spmv(m, rp, ci, v, x, 2.0, 0.0)
-    for variant in (0, 1, 5, 9):
+    for variant in (0, 1, 5, 9, PANEL):
         y, _ = run_variant(pls, m, n, rp, ci, v, x, 2.0, 0.0, np.zeros(m), variant=variant)
         assert rel_err(y, ref) <= TOL64
     lens = np.full(8, 9000)
@@ -154,7 +155,7 @@ def test_one_based_and_long_rows(pls, orc):
     ci2 = np.concatenate([np.sort(rng.choice(n, 9000, replace=False)) for _ in lens]).astype(np.int32)
     v2 = rng.standard_normal(int(rp2[-1]))
     ref2 = orc.spmv(8, rp2, ci2, v2, x)
-    for variant in (0, 1, 6):
+    for variant in (0, 1, 6, PANEL):
         y, _ = run_variant(pls, 8, n, rp2, ci2, v2, x, 1.0, 0.0, np.zeros(8), variant=variant)
         assert rel_err(y, ref2) <= TOL64
 
@@ -190,6 +191,39 @@ def test_sell_generic_kernel_order_matches_reference_kernel(pls, orc):
     ref = orc.scs_spmv(m, n, rp, ci, v, 16, 3200, x, 1.5, 0.25, y0.copy())
     y, _ = run_variant(pls, m, n, rp, ci, v, x, 1.5, 0.25, y0, sell=(16, 3200))
     assert rel_err(y, ref) <= TOL64
+
+
+@pytest.mark.parametrize("pw,wpc,window", [(3, 0, 0), (7, 2, 1), (12, 1, -1), (24, 1, 4)])
+def test_panel_shapes(pls, orc, pw, wpc, window):
+    """Column-panel variant: many narrow panels, few resident workgroups and
+    tiles capped by the packed-index width (several rounds per workgroup)."""
+    rng = np.random.default_rng(pw * 31 + wpc)
+    m, n = (600_000, 40_000) if pw == 24 else (5000, 3000)
+    rp, ci, v = powerlawish(rng, m, n, mean=3.0 if pw == 24 else 9.0, empty=0.2)
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    A = pls.SpMat.csr(m, n, rp, ci, v)
+    A.force_panel(pw, wpc, window)
+    A.optimize()
+    assert A.variant() == PANEL
+    info = A.panel_info()
+    assert info["panel_log2"] == pw
+    assert info["npanels"] == max(1, -(-n // (1 << pw)))
+    assert info["ntiles"] % 4 == 0 and info["rows_per_tile_max"] << pw <= 1 << 32
+    if pw == 24:
+        assert info["ntiles"] > 4 * info["grid"]  # more than one round of tiles
+    for alpha, beta in ((1.0, 0.0), (-0.75, 1.5)):
+        y = y0.copy()
+        A.spmv(x, y, alpha, beta)
+        assert rel_err(y, orc.spmv(m, rp, ci, v, x, alpha, beta, y0)) <= TOL64
+    y = np.full(m, np.nan)
+    A.spmv(x, y, 1.0, 0.0)
+    assert rel_err(y, orc.spmv(m, rp, ci, v, x)) <= TOL64
+    # fixed summation order: repeated launches give identical bits
+    for _ in range(3):
+        y2 = np.full(m, np.nan)
+        A.spmv(x, y2, 1.0, 0.0)
+        assert np.array_equal(y2, y)
 
 
 # ---------------------------------------------------------------- semantics
@@ -286,7 +320,7 @@ def to_bf16(a):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-@pytest.mark.parametrize("variant", [0, 1, 9, 10])
+@pytest.mark.parametrize("variant", [0, 1, 9, 10, PANEL])
 def test_reduced_precision(pls, orc, precision, variant):
     rng = np.random.default_rng(77)
     m, n = 4096, 4096
@@ -335,7 +369,7 @@ def test_powerlaw_4m_full(pls, orc, gen):
     x = gen.uniform(n, 0x5EED1002)
     ref = orc.spmv(n, rp, ci, v, x)
     A = pls.SpMat.csr(n, n, rp, ci, v)
-    for variant in (1, 9):
+    for variant in (1, PANEL, 9):
         A.force_variant(variant)
         A.optimize()
         y = np.zeros(n)
