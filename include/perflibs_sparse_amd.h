@@ -93,11 +93,10 @@ perflibs_status_t perflibs_amd_force_sell(perflibs_spmat_t A, perflibs_int_t C,
                                           perflibs_int_t sigma);
 
 /* Force the column-panel variant with panels of 2^panel_log2 columns
- * (1 <= panel_log2 <= 24, or 0 for the default), at most wg_per_cu resident
- * workgroups of 4 wave tiles per CU (1..16, or 0 for the default) and a pacing
- * window of `window` panels (0: waves never wait; -1: the default). */
+ * (1 <= panel_log2 <= 24, or 0 for the default) and at most wg_per_cu resident
+ * workgroups of 4 wave tiles per CU (1..16, or 0 for the default). */
 perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t panel_log2,
-                                           perflibs_int_t wg_per_cu, perflibs_int_t window);
+                                           perflibs_int_t wg_per_cu);
 /* Shape of a column-panel plan (for tests and tuning). */
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,

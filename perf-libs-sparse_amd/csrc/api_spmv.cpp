@@ -264,16 +264,13 @@ perflibs_status_t perflibs_amd_force_variant(perflibs_spmat_t A, int variant) {
   A->forced_sigma = -1;
   A->forced_panel_log2 = -1;
   A->forced_panel_wpc = -1;
-  A->forced_panel_window = -1;
   return PERFLIBS_STATUS_SUCCESS;
 }
 
 perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t panel_log2,
-                                           perflibs_int_t wg_per_cu, perflibs_int_t window) {
-  if (!A || panel_log2 < 0 || panel_log2 > 24 || wg_per_cu < 0 || wg_per_cu > 16 ||
-      window < -1 || window > 64)
+                                           perflibs_int_t wg_per_cu) {
+  if (!A || panel_log2 < 0 || panel_log2 > 24 || wg_per_cu < 0 || wg_per_cu > 16)
     return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
-  A->forced_panel_window = window;
   A->forced_variant = PERFLIBS_AMD_CSR_PANEL;
   A->forced_C = -1;
   A->forced_sigma = -1;

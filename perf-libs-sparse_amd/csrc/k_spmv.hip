@@ -272,21 +272,27 @@ __global__ __launch_bounds__(kBlock) void k_bsr_generic(
 // ---------------------------------------------------------------------------
 // Column-panel CSR: the same products and per-row sums as the CSR kernels,
 // reordered for x locality.  Each wave owns a tile of contiguous rows for the
-// whole launch and keeps their partial sums in a wave-private LDS array; it
-// sweeps the column panels in order (panel p = columns [p*2^pw, (p+1)*2^pw)),
-// streaming the tile's non-zeros of panel p (stored contiguously, rows in
-// order) and gathering x only from that panel.  The grid is persistent (every
-// workgroup resident), so at any moment the waves of an XCD gather from about
-// the same panel and its slice of x stays in that XCD's L2.  Correctness never
-// depends on that: waves do not wait for each other.
+// whole launch and keeps their partial sums in a wave-private LDS array.  The
+// tile's non-zeros are stored panel by panel (panel p = columns
+// [p*2^pw, (p+1)*2^pw)), rows in order inside a panel, and the wave streams
+// them front to back in steps of 256 entries, so at any moment it gathers x
+// from about one panel.  The grid is persistent (every workgroup resident)
+// and the tiles are balanced, so the waves of an XCD move through the panels
+// together and the current slice of x stays in that XCD's L2.  Correctness
+// never depends on that: waves do not wait for each other.
 //
-// Each lane takes a quad of 4 consecutive entries (one 16-byte index load).
-// A row's run of equal keys is summed inside the lane; runs that cross lanes
-// are joined through the previous lane's tail sum (a segmented scan over the
-// lanes when a run covers whole lanes).  Every row run of a 256-entry step is
-// added to its LDS accumulator by exactly one lane, so no two lanes update the
-// same row in one instruction and the summation order is fixed: results are
-// reproducible run to run, without atomics.
+// Each lane takes a quad of 4 consecutive entries (one 16-byte index load;
+// the builder pads every (tile, panel) run of entries to whole quads with a
+// packed index whose row field is all ones, which never names a tile row).
+// Entries carry the key (panel - first panel of the step) << 16 | row, which
+// is sorted along a step.  A run of equal keys is summed inside the lane;
+// runs that cross lanes are joined through the previous lane's tail sum (a
+// segmented scan over the lanes when a run covers whole lanes).  Each run
+// ends in exactly one lane, which adds the run to the row's LDS accumulator.
+// A row can own one run per panel of a step, so a step that spans several
+// panels adds the runs of one panel at a time: no two lanes update the same
+// row in one instruction, and the summation order is fixed (results are
+// reproducible run to run) without atomics.
 constexpr uint32_t kPanelNoRow = 0xffffffffu;
 constexpr int kPanelWaves = kBlock / 64;  // wave tiles per workgroup
 
@@ -310,20 +316,22 @@ __device__ __forceinline__ void ld_quad(const uint16_t *p, int64_t q, float (&a)
   a[2] = __uint_as_float(v.y << 16); a[3] = __uint_as_float(v.y & 0xffff0000u);
 }
 
-// Adds a lane's quad (row keys k, sorted, kPanelNoRow for masked entries;
-// products v) to the row accumulators; all 64 lanes call it together.
+// Adds a lane's quad (keys k, sorted along the step, kPanelNoRow for masked
+// entries; products v) to the row accumulators; all 64 lanes call it
+// together.  nparts = number of panels the step spans (wave-uniform).
 template <class A>
 __device__ __forceinline__ void panel_quad(A *acc, const uint32_t (&k)[4], const A (&v)[4],
-                                           int lane) {
+                                           int lane, int nparts) {
   const uint32_t kf = k[0], kl = k[3];
-  // head run (key kf), tail run (key kl); one run when the lane holds one row
+  // head run (key kf), tail run (key kl); one run when the lane holds one key
   const A H = ((v[0] + (k[1] == kf ? v[1] : A(0))) + (k[2] == kf ? v[2] : A(0))) +
               (k[3] == kf ? v[3] : A(0));
   const A T = (((k[0] == kl ? v[0] : A(0)) + (k[1] == kl ? v[1] : A(0))) +
                (k[2] == kl ? v[2] : A(0))) + v[3];
   // runs strictly inside the quad (entry 1, possibly with 2; entry 2 alone)
-  if (k[1] != kf && k[1] != kl) acc[k[1]] += v[1] + (k[2] == k[1] ? v[2] : A(0));
-  if (k[2] != kf && k[2] != kl && k[2] != k[1]) acc[k[2]] += v[2];
+  const bool m1 = k[1] != kf && k[1] != kl;
+  const A M1 = v[1] + (k[2] == k[1] ? v[2] : A(0));
+  const bool m2 = k[2] != kf && k[2] != kl && k[2] != k[1];
   const uint32_t kl_prev = __shfl_up(kl, 1);
   const uint32_t kf_next = __shfl_down(kf, 1);
   const uint64_t heads = __ballot(lane == 0 || kl_prev != kl);
@@ -338,102 +346,66 @@ __device__ __forceinline__ void panel_quad(A *acc, const uint32_t (&k)[4], const
     }
   }
   const A s_prev = __shfl_up(S, 1);
-  if (kf != kl) acc[kf] += H + (lane > 0 && kl_prev == kf ? s_prev : A(0));
-  if (kl != kPanelNoRow && (lane == 63 || kf_next != kl)) acc[kl] += S;
+  const bool hd = kf != kl;
+  const A Hc = H + (lane > 0 && kl_prev == kf ? s_prev : A(0));
+  const bool tl = kl != kPanelNoRow && (lane == 63 || kf_next != kl);
+  for (int part = 0; part < nparts; ++part) {
+    const uint32_t pt = uint32_t(part);
+    if (m1 && (k[1] >> 16) == pt) acc[k[1] & 0xffffu] += M1;
+    if (m2 && (k[2] >> 16) == pt) acc[k[2] & 0xffffu] += v[2];
+    if (hd && (kf >> 16) == pt) acc[kf & 0xffffu] += Hc;
+    if (tl && (kl >> 16) == pt) acc[kl & 0xffffu] += S;
+  }
 }
-
-// Pacing of the panel sweep (a locality hint only).  Each XCD counts, per
-// panel, the waves that have finished it (one 128-byte line per counter,
-// counts accumulate over launches: launch `epoch` is done with panel p when
-// its counter reaches (epoch + 1) * waves of the XCD).  Before panel p a wave
-// waits, for a bounded number of short sleeps, until its XCD has finished
-// panel p - window; after one wait that runs out it stops waiting, so a wave
-// never depends on another for progress and every wave exits.
-constexpr int kPanelSyncStride = 32;     // uint32 per counter (128 bytes)
-constexpr int kPanelSpinLimit = 2048;    // s_sleep(1) rounds per wait, at most
-
-__device__ __forceinline__ uint32_t panel_xcd(uint32_t b) { return b & 7u; }
 
 template <class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_csr_panel(
     int32_t ngroups, int32_t npanels, int32_t pw, int32_t rcap,
     const int32_t *__restrict__ tile_row0, const int64_t *__restrict__ seg,
     const uint32_t *__restrict__ idx, const V *__restrict__ val,
-    const X *__restrict__ x, X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta,
-    uint32_t *__restrict__ sync, int32_t window, uint32_t epoch) {
+    const X *__restrict__ x, X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta) {
   using A = acc_t<X>;
   extern __shared__ __align__(16) unsigned char panel_lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   A *acc = reinterpret_cast<A *>(panel_lds) + wave * rcap;
   const uint32_t cmask = (1u << pw) - 1u;
-  constexpr int U = 2;  // 256-entry steps in flight per wave
-  // pacing only for a single round of resident workgroups
-  const bool paced = window > 0 && ngroups <= int32_t(gridDim.x) && npanels > 1;
-  bool pace = paced;
-  const uint32_t xcd = panel_xcd(blockIdx.x);
-  const uint32_t wg_in_xcd = (gridDim.x >> 3) + (xcd < (gridDim.x & 7u) ? 1u : 0u);
-  const uint32_t target = (epoch + 1u) * (wg_in_xcd * kPanelWaves);
-  uint32_t *xsync = sync + size_t(xcd) * npanels * kPanelSyncStride;
   for (int32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const int32_t t = g * kPanelWaves + wave;
-    const int32_t r0 = tile_row0[t], nr = tile_row0[t + 1] - r0;
-    for (int32_t r = lane; r < nr; r += 64) acc[r] = A(0);
+    const int32_t r0 = tile_row0[t];
+    const uint32_t nr = uint32_t(tile_row0[t + 1] - r0);
+    for (uint32_t r = lane; r < nr; r += 64) acc[r] = A(0);
     const int64_t *sg = seg + int64_t(t) * npanels;
-    int64_t e0 = sg[0];
-    for (int32_t p = 0; p < npanels; ++p) {
-      if (pace && p >= window) {
-        uint32_t *c = xsync + size_t(p - window) * kPanelSyncStride;
-        int spins = 0;
-        while (int32_t(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-          if (++spins > kPanelSpinLimit) {
-            pace = false;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
+    const int64_t te = sg[npanels] & ~int64_t(3);  // end of the tile's entries
+    int32_t p = 0;                                 // panel holding the step's first quad
+    for (int64_t b = sg[0] & ~int64_t(3); b < te; b += 256) {
+      while ((sg[p + 1] & ~int64_t(3)) <= b) ++p;
+      int32_t pe = p;  // last panel with a quad in this step
+      while (pe + 1 < npanels && (sg[pe + 1] & ~int64_t(3)) <= min(b + 252, te - 4)) ++pe;
+      const int64_t q = b + 4 * lane;
+      const int64_t qc = min(q, te - 4);
+      const u32x4_t id = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(idx + qc));
+      A a[4];
+      ld_quad(val, qc, a);
+      int32_t pl = p;  // panel of this lane's quad
+      for (int32_t k = p + 1; k <= pe; ++k) pl += qc >= (sg[k] & ~int64_t(3)) ? 1 : 0;
+      const X *xp = x + (int64_t(pl) << pw);
+      const uint32_t ids[4] = {id.x, id.y, id.z, id.w};
+      A xv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xv[i] = A(xp[ids[i] & cmask]);
+      const uint32_t part = uint32_t(pl - p) << 16;
+      uint32_t kk[4];
+      A vv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t row = ids[i] >> pw;
+        const bool ok = q < te && row < nr;
+        kk[i] = ok ? part | row : kPanelNoRow;
+        vv[i] = ok ? a[i] * xv[i] : A(0);
       }
-      const int64_t e1 = sg[p + 1];
-      const int64_t s0 = e0 & ~int64_t(3), s1p = e1 & ~int64_t(3);
-      const int64_t s1 = s1p - (e0 & 3);  // end of the real entries
-      const X *xp = x + (int64_t(p) << pw);
-      for (int64_t b = s0; b < s1p; b += U * 256) {
-        u32x4_t id[U];
-        A a[U][4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int64_t q = min(b + u * 256 + 4 * lane, s1p - 4);
-          id[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(idx + q));
-          ld_quad(val, q, a[u]);
-        }
-        A xv[U][4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          xv[u][0] = A(xp[id[u].x & cmask]);
-          xv[u][1] = A(xp[id[u].y & cmask]);
-          xv[u][2] = A(xp[id[u].z & cmask]);
-          xv[u][3] = A(xp[id[u].w & cmask]);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int64_t q0 = b + u * 256 + 4 * lane;
-          const uint32_t ids[4] = {id[u].x, id[u].y, id[u].z, id[u].w};
-          uint32_t kk[4];
-          A vv[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const bool ok = q0 + i < s1;
-            kk[i] = ok ? ids[i] >> pw : kPanelNoRow;
-            vv[i] = ok ? a[u][i] * xv[u][i] : A(0);
-          }
-          panel_quad(acc, kk, vv, lane);
-        }
-      }
-      e0 = e1;
-      if (paced && lane == 0)
-        __hip_atomic_fetch_add(xsync + size_t(p) * kPanelSyncStride, 1u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+      panel_quad(acc, kk, vv, lane, pe - p + 1);
     }
-    for (int32_t r = lane; r < nr; r += 64) {
+    for (uint32_t r = lane; r < nr; r += 64) {
       A res = alpha * acc[r];
       if (BETA) res += beta * A(y[r0 + r]);
       y[r0 + r] = X(res);
@@ -644,18 +616,16 @@ hipError_t csr_panel(const pls::DevPanel &A, bool dbl, double alpha, const void 
     using X = std::remove_pointer_t<decltype(xtag)>;
     const size_t lds = size_t(kPanelWaves) * A.rcap * sizeof(acc_t<X>);
     const int32_t ngroups = A.ntiles / kPanelWaves;
-    const int32_t window = A.sync.get() ? A.window : 0;
-    const uint32_t epoch = A.launches++;
     if (beta != 0.0)
       k_csr_panel<V, X, true><<<A.grid, kBlock, lds, s>>>(
           ngroups, A.npanels, A.pw_log2, A.rcap, A.tile_row0.as<int32_t>(), A.seg.as<int64_t>(),
           A.idx.as<uint32_t>(), A.vals.as<V>(), (const X *)x, (X *)y, acc_t<X>(alpha),
-          acc_t<X>(beta), A.sync.as<uint32_t>(), window, epoch);
+          acc_t<X>(beta));
     else
       k_csr_panel<V, X, false><<<A.grid, kBlock, lds, s>>>(
           ngroups, A.npanels, A.pw_log2, A.rcap, A.tile_row0.as<int32_t>(), A.seg.as<int64_t>(),
           A.idx.as<uint32_t>(), A.vals.as<V>(), (const X *)x, (X *)y, acc_t<X>(alpha),
-          acc_t<X>(0), A.sync.as<uint32_t>(), window, epoch);
+          acc_t<X>(0));
     return hipGetLastError();
   });
 }

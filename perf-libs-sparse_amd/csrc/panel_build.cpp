@@ -38,12 +38,14 @@ template <class F> void parallel_chunks(int64_t n, F &&fn) {
 bool panel_build(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci, const void *vals,
                  DType dt, int32_t want_tiles, int32_t rcap, int32_t pw_log2, PanelHost &P) {
   if (m <= 0 || rcap <= 0 || pw_log2 < 1 || pw_log2 > 30) return false;
-  if ((int64_t(rcap) << pw_log2) > (int64_t(1) << 32)) return false;  // packed index width
+  // the row field must hold rows 0..rcap-1 and the all-ones padding marker
+  if (int64_t(rcap) >= (int64_t(1) << (32 - pw_log2)) || rcap > 65536) return false;
   P = PanelHost{};
   P.m = m;
   P.n = n;
   P.pw_log2 = pw_log2;
   P.npanels = std::max<int32_t>(1, int32_t((int64_t(n) + (int64_t(1) << pw_log2) - 1) >> pw_log2));
+  if (P.npanels > 65536) return false;  // the kernel keys a step's panels in 16 bits
   const int32_t base = rp[0];
   const int64_t nnz = int64_t(rp[m]) - base;
 
@@ -91,8 +93,8 @@ bool panel_build(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci, con
   // Segments are padded to whole quads (4 entries) so a lane can load four
   // consecutive entries with one 16-byte index load; seg[k] holds the padded
   // start of segment k (a multiple of 4) plus, in its low 2 bits, how many of
-  // that segment's entries are padding.  Padding repeats the segment's last
-  // packed index with value 0.
+  // that segment's entries are padding.  A padding entry has value 0 and the
+  // packed index with an all-ones row field and column 0 of its panel.
   P.seg.assign(size_t(P.ntiles) * np + 1, 0);
   int64_t at = 0;
   for (size_t k = 0; k < count.size(); ++k) {
@@ -117,10 +119,11 @@ bool panel_build(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci, con
           P.idx[size_t(e)] = (uint32_t(row - row0) << pw_log2) | uint32_t(c - (int32_t(p) << pw_log2));
           std::memcpy(&P.vals[size_t(e) * vsz], static_cast<const uint8_t *>(vals) + size_t(j) * vsz, vsz);
         }
+      const uint32_t pad_idx = ~((1u << pw_log2) - 1u);
       for (int64_t p = 0; p < np; ++p) {  // padding entries
         const size_t k = size_t(t * np + p);
         const int64_t npad = P.seg[k] & 3;
-        for (int64_t e = fill[size_t(p)]; e < fill[size_t(p)] + npad; ++e) P.idx[size_t(e)] = P.idx[size_t(e - 1)];
+        for (int64_t e = fill[size_t(p)]; e < fill[size_t(p)] + npad; ++e) P.idx[size_t(e)] = pad_idx;
       }
     }
   });

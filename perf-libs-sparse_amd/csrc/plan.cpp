@@ -22,7 +22,6 @@ namespace pls {
 namespace {
 constexpr int kAdaNnz = 2048;       // must equal plk::kAdaNnz (k_spmv.hip)
 constexpr int kAdaMaxRows = 1024;   // rows per adaptive block
-constexpr int kPanelWindow = 0;     // column-panel pacing window (panels)
 
 struct SellParams {
   int32_t C, sigma;
@@ -260,10 +259,13 @@ perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A) {
                     PERFLIBS_SPARSE_OPERATION_NOTRANS, variant);
 }
 
-// Column-panel plan.  Shape: panels of 2^pw columns (1 MiB of x by default),
-// and as many wave tiles as the GPU keeps resident at the largest occupancy
-// (workgroups per CU, at most 8) whose LDS accumulators hold 1.25x the mean
-// rows per tile; tiles beyond the resident count run in further rounds.
+// Column-panel plan.  Shape: panels of 2^pw columns (512 KiB of x by
+// default, a slice that several panels' worth of drift between waves still
+// keeps inside a 4 MiB XCD L2), and as many wave tiles as the GPU keeps
+// resident at up to 4 workgroups per CU (16 waves: the LDS then holds
+// 1.25x the mean rows per tile for 4M-row FP64 operands, and the panel sweep
+// measured fastest there); tiles beyond the resident count run in further
+// rounds.
 static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
                                           int variant) {
   const bool dbl = A->dt == DType::F64;
@@ -274,8 +276,10 @@ static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return hip_fail(A, e, "device query");
   cus = std::max(cus, 1);
-  const int32_t pw = A->forced_panel_log2 > 0 ? A->forced_panel_log2 : (dbl ? 17 : 18);
-  const int32_t rmax_idx = int32_t(std::min<int64_t>(int64_t(1) << (32 - pw), 1 << 20));
+  int32_t pw = A->forced_panel_log2 > 0 ? A->forced_panel_log2 : (dbl ? 16 : 17);
+  while (pw < 30 && ((int64_t(o.n) - 1) >> pw) >= 65536) ++pw;  // at most 65536 panels
+  // row field of the packed index: tile rows plus the all-ones padding marker
+  const int32_t rmax_idx = int32_t(std::min<int64_t>((int64_t(1) << (32 - pw)) - 64, 1 << 16));
   constexpr size_t kMaxLds = 64 * 1024;  // per workgroup
   const int32_t rmax_lds = int32_t(kMaxLds / (4 * abytes));
   const int32_t rmax = std::min(rmax_idx, rmax_lds);
@@ -312,13 +316,6 @@ static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand
   d->rcap = rcap;
   d->grid = int32_t(std::min<int64_t>(P.ntiles / 4, int64_t(cus) * wpc));
   d->vs = vs;
-  d->window = A->forced_panel_window >= 0 ? A->forced_panel_window : kPanelWindow;
-  if (d->npanels > 1 && d->window > 0) {
-    const size_t sb = size_t(8) * d->npanels * 128;
-    e = d->sync.alloc(sb);
-    if (e == hipSuccess) e = hipMemset(d->sync.get(), 0, sb);
-    if (e != hipSuccess) return hip_fail(A, e, "column-panel plan upload");
-  }
   e = d->tile_row0.upload(P.tile_row0.data(), sizeof(int32_t) * P.tile_row0.size());
   if (e == hipSuccess) e = d->seg.upload(P.seg.data(), sizeof(int64_t) * P.seg.size());
   if (e == hipSuccess) e = d->idx.upload(P.idx.data(), sizeof(uint32_t) * P.idx.size());

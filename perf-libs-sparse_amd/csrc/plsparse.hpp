@@ -136,12 +136,6 @@ struct DevPanel {
   DevBuf seg;        // int64 [ntiles*npanels+1], quad-aligned start | padding count
   DevBuf idx;        // uint32 [padded nnz], (row - tile row0) << pw_log2 | col - panel col0
   DevBuf vals;       // [padded nnz]
-  // Pacing of the panel sweep: per-XCD, per-panel counters of finished waves
-  // (8 * npanels lines of 128 bytes, zeroed at build), the window in panels
-  // (0: no pacing) and the number of launches so far.
-  DevBuf sync;
-  int32_t window = 0;
-  mutable uint32_t launches = 0;
 };
 
 struct DevBsr {
@@ -196,7 +190,7 @@ struct perflibs_spmat_top_t {
   int select_mode = -1;   // -1: learned when a model is loaded, else heuristic
   int forced_variant = -1;
   int forced_C = -1, forced_sigma = -1;
-  int forced_panel_log2 = -1, forced_panel_wpc = -1, forced_panel_window = -1;  // column-panel shape overrides
+  int forced_panel_log2 = -1, forced_panel_wpc = -1;  // column-panel shape overrides
   int precision = PERFLIBS_AMD_VALUES_NATIVE;
 
   pls::DevBuf stage_x, stage_y;  // staging of host x / y
@@ -257,8 +251,9 @@ struct PanelHost {
   std::vector<uint8_t> vals;       // [padded nnz] values of the handle dtype
 };
 // Wave tiles of at most rcap rows balanced to about nnz/want_tiles non-zeros,
-// panels of 2^pw_log2 columns; 0-based CSR input.  False if the packed index
-// (rcap << pw_log2) does not fit 32 bits.
+// panels of 2^pw_log2 columns; 0-based CSR input.  False unless the packed
+// row field holds every tile row plus the all-ones padding marker
+// (rcap < 2^(32 - pw_log2)), rcap <= 65536 and there are at most 65536 panels.
 bool panel_build(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci, const void *vals,
                  DType dt, int32_t want_tiles, int32_t rcap, int32_t pw_log2, PanelHost &P);
 

@@ -67,12 +67,15 @@ perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, fl
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-// Hand rules on the same features: dense blocks -> BSR, strongly irregular
+// Hand rules on the same features: dense blocks -> BSR, columns scattered
+// over an x much larger than an XCD's L2 -> column panels, strongly irregular
 // rows -> CSR-adaptive, otherwise SELL-C-sigma with a moderate sort window.
 int heuristic_variant(const float *f, const Operand &o) {
   const double fill4 = f[9], fill8 = f[10], imbalance = f[6], mean = std::expm1(f[3]);
+  const double spread = f[8];  // mean column span of a row / n
   if (fill8 >= 0.9 && o.m % 8 == 0 && o.n % 8 == 0) return PERFLIBS_AMD_BSR_B8;
   if (fill4 >= 0.9 && o.m % 4 == 0 && o.n % 4 == 0) return PERFLIBS_AMD_BSR_B4;
+  if (o.n >= (1 << 20) && o.m >= (1 << 18) && spread > 0.25) return PERFLIBS_AMD_CSR_PANEL;
   if (imbalance > std::log(8.0)) return PERFLIBS_AMD_CSR_ADAPTIVE;
   return mean >= 24 ? PERFLIBS_AMD_SELL_C64_S1024 : PERFLIBS_AMD_SELL_C32_S1024;
 }

@@ -94,7 +94,7 @@ _SIGS = {
     "perflibs_amd_get_variant": (_I, [_P, _P]),
     "perflibs_amd_set_precision": (_I, [_P, C.c_int]),
     "perflibs_amd_force_sell": (_I, [_P, _I, _I]),
-    "perflibs_amd_force_panel": (_I, [_P, _I, _I, _I]),
+    "perflibs_amd_force_panel": (_I, [_P, _I, _I]),
     "perflibs_amd_panel_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
@@ -247,8 +247,8 @@ class SpMat:
     def force_sell(self, Cc: int, sigma: int):
         check("force_sell", lib().perflibs_amd_force_sell(self.h, Cc, sigma))
 
-    def force_panel(self, panel_log2: int = 0, wg_per_cu: int = 0, window: int = -1):
-        check("force_panel", lib().perflibs_amd_force_panel(self.h, panel_log2, wg_per_cu, window))
+    def force_panel(self, panel_log2: int = 0, wg_per_cu: int = 0):
+        check("force_panel", lib().perflibs_amd_force_panel(self.h, panel_log2, wg_per_cu))
 
     def panel_info(self) -> dict:
         v = [C.c_int32() for _ in range(5)]
