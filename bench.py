@@ -44,6 +44,19 @@ def algorithmic_bytes(n: int, nnz: int) -> int:
     return nnz * (8 + 4) + (n + 1) * 4 + n * 8 + n * 8
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summaries (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    factor the same counter reads on a device copy of known size), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                parts = line.split()
+                if len(parts) == 4 and parts[1] == kernel and parts[2] == workload:
+                    return int(parts[3]), f.name
+    return None
+
+
 def time_gpu(A, xd, yd, warmup: int, steps: int):
     import torch
 
@@ -193,6 +206,8 @@ def main() -> int:
     regret = (sweep[chosen] / sweep[best] - 1.0) if chosen in sweep else None
 
     cpu = None if args.no_cpu else cpu_baseline(n, rp, ci, v, x, y_gpu)
+    workload = "spmv_fp64_powerlaw_4m" if n == N_ROWS else f"spmv_fp64_powerlaw_{n}"
+    traffic = measured_traffic(chosen, workload)
 
     out = {
         "metric": METRIC,
@@ -208,14 +223,16 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "spmv_fp64_powerlaw_4m", "rows": n, "cols": n, "nnz": nnz,
+            "workload": workload, "rows": n, "cols": n, "nnz": nnz,
             "row_length": "round(lognormal(2.5, 1.0)) clipped to [1, 20000]",
             "alpha": 1.0, "beta": 0.0, "seed": SEED_A, "variant": chosen,
             "hbm_fraction": round(value / HBM_PEAK_GBS, 4),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic[0] if traffic else None,
+            "traffic_source": f"profiles/{traffic[1]}" if traffic else None,
             "kernel": chosen, "kernel_ms": round(kern_ms, 4), "algorithmic_bytes": nbytes,
         },
         "cpu_baseline": cpu,
