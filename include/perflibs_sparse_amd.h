@@ -164,6 +164,43 @@ perflibs_status_t perflibs_amd_sell_export(perflibs_spmat_t A, int64_t *cs,
                                            int64_t *col_indx_min,
                                            int64_t *offsets, double *vals);
 
+/* ---- batched products: one persistent launch, replayed from a hipGraph ----
+ * A batch holds nprob independent products Y_p = alpha*A_p*X_p + beta*Y_p:
+ * A_p a sparse handle (CSR, or BSR / SELL-C-sigma through their CSR view),
+ * NOTRANS, all of one data type; X_p (n_p x k_p) and Y_p (m_p x k_p) row-major
+ * device arrays, k_p in {1, 2, 4, 8, 16} (k_p = 1: SpMV).  beta == 0 never
+ * reads Y; alpha == 0 gives beta*Y.  The matrices are copied to HBM at
+ * create; later changes to the handles are not seen. */
+typedef struct perflibs_amd_batch_s *perflibs_amd_batch_t;
+/* precision: PERFLIBS_AMD_VALUES_NATIVE, or PERFLIBS_AMD_VALUES_BF16 for _s
+ * handles (BF16 values, FP32 X/Y and accumulation). */
+perflibs_status_t perflibs_amd_batch_create(perflibs_amd_batch_t *B, int64_t nprob,
+                                            const perflibs_spmat_t *A, const perflibs_int_t *k,
+                                            int precision);
+perflibs_status_t perflibs_amd_batch_destroy(perflibs_amd_batch_t B);
+/* Device pointers of every X_p and Y_p (arrays of nprob pointers in host
+ * memory).  May be called again between executions; no recapture needed. */
+perflibs_status_t perflibs_amd_batch_set_operands(perflibs_amd_batch_t B, const void *const *X,
+                                                  void *const *Y);
+/* autotune != 0: time candidate tile sizes and per-k row shapes by graph
+ * replays (reading the current X, writing scratch outputs) and keep the
+ * fastest; autotune == 0 keeps the current shape. */
+perflibs_status_t perflibs_amd_batch_optimize(perflibs_amd_batch_t B, int autotune);
+/* Set the shape by hand: tile_nnz non-zeros per tile at most, and for each k
+ * class (k = 1, 2, 4, 8, 16) lanes per row (a power of two in [k/cpl, 64]) and
+ * RHS columns per lane cpl (1, 2 or 4, at most k).  lanes / cpl may be NULL. */
+perflibs_status_t perflibs_amd_batch_set_shape(perflibs_amd_batch_t B, perflibs_int_t tile_nnz,
+                                               const perflibs_int_t *lanes,
+                                               const perflibs_int_t *cpl);
+/* Enqueue the whole batch on the library stream as one kernel launch, replayed
+ * from a hipGraph (captured on first use and after a change of alpha, beta or
+ * shape).  Not synchronized. */
+perflibs_status_t perflibs_amd_batch_exec(perflibs_amd_batch_t B, double alpha, double beta);
+/* Current shape: tiles, tile_nnz, resident workgroups, lanes[5], cpl[5]. */
+perflibs_status_t perflibs_amd_batch_info(perflibs_amd_batch_t B, int64_t *ntiles,
+                                          perflibs_int_t *tile_nnz, perflibs_int_t *grid,
+                                          perflibs_int_t *lanes, perflibs_int_t *cpl);
+
 /* Last error message of the library (thread-unsafe, for diagnostics). */
 const char *perflibs_amd_last_error(void);
 

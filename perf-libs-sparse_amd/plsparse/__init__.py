@@ -106,6 +106,13 @@ _SIGS = {
     "perflibs_amd_mlp_adam": (_I, [_P, _P, _P, _P, C.c_int, _F, _F, _F, _F]),
     "perflibs_amd_sell_info": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_sell_export": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_batch_create": (_I, [_PP, C.c_int64, _P, _P, C.c_int]),
+    "perflibs_amd_batch_destroy": (_I, [_P]),
+    "perflibs_amd_batch_set_operands": (_I, [_P, _P, _P]),
+    "perflibs_amd_batch_optimize": (_I, [_P, C.c_int]),
+    "perflibs_amd_batch_set_shape": (_I, [_P, _I, _P, _P]),
+    "perflibs_amd_batch_exec": (_I, [_P, _D, _D]),
+    "perflibs_amd_batch_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "perflibs_amd_last_error": (C.c_char_p, []),
 }
 
@@ -373,3 +380,61 @@ def spmm_optimize(transA, transB, alpha_hint, A: SpMat, B: SpMat, beta_hint, Cm:
     if raise_on_error:
         check("perflibs_spmm_optimize", st)
     return st
+
+
+class Batch:
+    """A batch of CSR products Y_p = alpha*A_p*X_p + beta*Y_p run as one
+    persistent launch replayed from a hipGraph (perflibs_amd_batch_*)."""
+
+    K_CLASSES = (1, 2, 4, 8, 16)
+
+    def __init__(self, mats, ks, precision=VALUES_NATIVE):
+        self.mats = list(mats)  # keep the handles alive while the batch exists
+        n = len(self.mats)
+        hs = (C.c_void_p * max(n, 1))(*[m.h for m in self.mats])
+        kk = np.ascontiguousarray(ks, dtype=np.int32)
+        out = C.c_void_p()
+        check("batch_create", lib().perflibs_amd_batch_create(C.byref(out), n, hs, kk.ctypes.data, precision))
+        self.h = out.value
+        self._ops = None
+
+    def set_operands(self, X_ptrs, Y_ptrs):
+        n = len(self.mats)
+        xs = (C.c_void_p * max(n, 1))(*[int(p) for p in X_ptrs])
+        ys = (C.c_void_p * max(n, 1))(*[int(p) for p in Y_ptrs])
+        check("batch_set_operands", lib().perflibs_amd_batch_set_operands(self.h, xs, ys))
+        self._ops = (xs, ys)
+
+    def optimize(self, autotune=True):
+        check("batch_optimize", lib().perflibs_amd_batch_optimize(self.h, int(bool(autotune))))
+
+    def set_shape(self, tile_nnz, lanes=None, cpl=None):
+        la = None if lanes is None else np.ascontiguousarray(lanes, dtype=np.int32)
+        ca = None if cpl is None else np.ascontiguousarray(cpl, dtype=np.int32)
+        check("batch_set_shape", lib().perflibs_amd_batch_set_shape(
+            self.h, tile_nnz, None if la is None else la.ctypes.data, None if ca is None else ca.ctypes.data))
+
+    def exec(self, alpha=1.0, beta=0.0):
+        check("batch_exec", lib().perflibs_amd_batch_exec(self.h, alpha, beta))
+
+    def info(self) -> dict:
+        nt = C.c_int64()
+        tn = C.c_int32()
+        gr = C.c_int32()
+        la = np.zeros(5, np.int32)
+        ca = np.zeros(5, np.int32)
+        check("batch_info", lib().perflibs_amd_batch_info(self.h, C.byref(nt), C.byref(tn), C.byref(gr),
+                                                          la.ctypes.data, ca.ctypes.data))
+        return {"ntiles": nt.value, "tile_nnz": tn.value, "grid": gr.value,
+                "lanes": dict(zip(self.K_CLASSES, la.tolist())), "cpl": dict(zip(self.K_CLASSES, ca.tolist()))}
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            lib().perflibs_amd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -180,3 +180,22 @@ def bsr_to_csr_rowmajor(brp, bci, bval, b: int):
             col[rp[r]:rp[r + 1]] = cols
             val[rp[r]:rp[r + 1]] = blocks[k0:k1, i, :].reshape(-1)
     return rp.astype(np.int32), col, val
+
+
+def banded_batch(nprob: int, seed: int, rows=(256, 65536), bw=(32, 512), nnz=(5, 40),
+                 ks=(1, 4, 16)):
+    """Problems of the batched configuration: square banded random matrices
+    with rows log-uniform in `rows`, bandwidth uniform in `bw`, non-zeros per
+    row uniform in `nnz` (distinct columns inside the band, N(0,1) values) and
+    a right-hand-side count drawn uniformly from `ks`.  Returns a list of
+    (m, row_ptr, col_indx, vals, k)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(rows[0]), np.log(rows[1])
+    out = []
+    for p in range(nprob):
+        m = int(round(float(np.exp(rng.uniform(lo, hi)))))
+        w = int(rng.integers(bw[0], bw[1] + 1))
+        k = int(ks[int(rng.integers(0, len(ks)))])
+        rp, ci, v = banded(m, w, nnz[0], nnz[1], (seed * 1_000_003 + p) & 0xFFFFFFFFFFFF)
+        out.append((m, rp, ci, v, k))
+    return out

@@ -1,0 +1,182 @@
+"""GPU parity of the batched products (one persistent launch replayed from a
+hipGraph) through the C-ABI: FP64 within 1e-12 of the compensated oracle per
+problem, FP32 / BF16 storage with FP32 accumulation against the exact result of
+the stored values and against the FP64 oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def to_bf16(a):
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def make(pls, gen, nprob, seed, rows=(64, 3000), ks=(1, 2, 4, 8, 16), dtype=np.float64,
+         precision=0, empty=True):
+    probs = gen.banded_batch(nprob, seed, rows=rows, ks=ks)
+    if empty:  # a problem whose rows are all empty
+        m = 77
+        probs.append((m, np.zeros(m + 1, np.int32), np.zeros(1, np.int32), np.zeros(1), 4))
+    mats = [pls.SpMat.csr(m, m, rp, ci, v.astype(dtype)) for (m, rp, ci, v, k) in probs]
+    B = pls.Batch(mats, [p[4] for p in probs], precision)
+    return probs, B
+
+
+def operands(probs, seed, dtype):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    X = [torch.from_numpy(rng.uniform(-1, 1, (m, k)).astype(dtype)).cuda() for (m, _, _, _, k) in probs]
+    Y0 = [rng.uniform(-1, 1, (m, k)).astype(dtype) for (m, _, _, _, k) in probs]
+    Y = [torch.from_numpy(y).cuda() for y in Y0]
+    return X, Y, Y0, tdt
+
+
+def oracle_of(orc, probs, X, alpha, beta, Y0, vals=None):
+    out = []
+    for i, (m, rp, ci, v, k) in enumerate(probs):
+        vv = v if vals is None else vals[i]
+        x = X[i].cpu().numpy().astype(np.float64)
+        out.append(orc.spmm(m, k, rp, ci, vv, x, alpha, beta, None if beta == 0 else Y0[i].astype(np.float64)))
+    return out
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-0.75, 1.5)])
+def test_batch_fp64_parity(pls, orc, gen, alpha, beta):
+    import torch
+
+    probs, B = make(pls, gen, 40, 11)
+    X, Y, Y0, _ = operands(probs, 12, np.float64)
+    if beta == 0.0:
+        for y in Y:
+            y.fill_(float("nan"))
+    B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    B.exec(alpha, beta)
+    torch.cuda.synchronize()
+    ref = oracle_of(orc, probs, X, alpha, beta, Y0)
+    for i, (m, _, _, _, k) in enumerate(probs):
+        got = Y[i].cpu().numpy()
+        assert rel_err(got, ref[i]) <= TOL64, (i, m, k)
+        if m == 77:
+            assert np.array_equal(got, beta * Y0[i]) or beta == 0.0 and not got.any()
+
+
+def test_batch_shapes_and_autotune(pls, orc, gen):
+    import torch
+
+    probs, B = make(pls, gen, 24, 21)
+    X, Y, Y0, _ = operands(probs, 22, np.float64)
+    B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    ref = oracle_of(orc, probs, X, 1.0, 0.0, Y0)
+    shapes = [  # (tile_nnz, lanes per k class, columns per lane per k class)
+        (64, [2, 2, 4, 8, 16], [1, 1, 1, 1, 1]),
+        (1024, [64, 32, 16, 8, 64], [1, 1, 1, 1, 1]),
+        (4096, [4, 1, 1, 2, 4], [1, 2, 4, 4, 4]),
+        (100000, [16, 2, 2, 4, 8], [1, 1, 2, 2, 2]),
+    ]
+    for tile, lanes, cpl in shapes:
+        B.set_shape(tile, lanes, cpl)
+        for y in Y:
+            y.fill_(float("nan"))
+        B.exec(1.0, 0.0)
+        torch.cuda.synchronize()
+        info = B.info()
+        assert info["tile_nnz"] == tile and list(info["lanes"].values()) == lanes
+        for i in range(len(probs)):
+            assert rel_err(Y[i].cpu().numpy(), ref[i]) <= TOL64, (tile, lanes, cpl, i)
+    # invalid shapes are rejected
+    for lanes, cpl in (([3, 2, 4, 8, 16], [1] * 5), ([2, 2, 4, 8, 8], [1] * 5), ([2, 2, 4, 8, 16], [1, 4, 1, 1, 1])):
+        with pytest.raises(pls.PerflibsError):
+            B.set_shape(2048, lanes, cpl)
+    B.optimize(autotune=True)
+    info = B.info()
+    assert info["ntiles"] > 0 and info["grid"] > 0
+    for y in Y:
+        y.fill_(float("nan"))
+    B.exec(1.0, 0.0)
+    torch.cuda.synchronize()
+    for i in range(len(probs)):
+        assert rel_err(Y[i].cpu().numpy(), ref[i]) <= TOL64
+
+
+def test_batch_graph_replays_and_new_operands(pls, orc, gen):
+    import torch
+
+    probs, B = make(pls, gen, 16, 31, ks=(1, 4, 16))
+    X, Y, Y0, _ = operands(probs, 32, np.float64)
+    B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    B.exec(2.0, 0.0)
+    torch.cuda.synchronize()
+    first = [y.cpu().numpy().copy() for y in Y]
+    ref = oracle_of(orc, probs, X, 2.0, 0.0, Y0)
+    for i in range(len(probs)):
+        assert rel_err(first[i], ref[i]) <= TOL64
+    for _ in range(3):  # replays of the same graph, eager work in between
+        for y in Y:
+            y.zero_()
+        torch.cuda.synchronize()
+        B.exec(2.0, 0.0)
+        torch.cuda.synchronize()
+        for i in range(len(probs)):
+            assert np.array_equal(Y[i].cpu().numpy(), first[i])
+    # new X buffers through the pointer table, same graph
+    X2 = [torch.flip(x, [0]).contiguous() for x in X]
+    B.set_operands([x.data_ptr() for x in X2], [y.data_ptr() for y in Y])
+    B.exec(2.0, 0.0)
+    torch.cuda.synchronize()
+    ref2 = oracle_of(orc, probs, X2, 2.0, 0.0, Y0)
+    for i in range(len(probs)):
+        assert rel_err(Y[i].cpu().numpy(), ref2[i]) <= TOL64
+    # alpha = 0 gives beta*Y and ignores X (NaN in X)
+    for x in X2:
+        x.fill_(float("nan"))
+    for i, y in enumerate(Y):
+        y.copy_(torch.from_numpy(Y0[i]))
+    B.exec(0.0, 0.5)
+    torch.cuda.synchronize()
+    for i in range(len(probs)):
+        assert np.array_equal(Y[i].cpu().numpy(), 0.5 * Y0[i])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_batch_reduced_precision(pls, orc, gen, precision):
+    import torch
+
+    probs, B = make(pls, gen, 24, 41, dtype=np.float32, precision=2 if precision == "bf16" else 0,
+                    empty=False)
+    X, Y, Y0, _ = operands(probs, 42, np.float32)
+    B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    B.exec(1.0, 0.0)
+    torch.cuda.synchronize()
+    stored = [to_bf16(v.astype(np.float32)) if precision == "bf16" else v.astype(np.float32)
+              for (_, _, _, v, _) in probs]
+    exact_stored = oracle_of(orc, probs, X, 1.0, 0.0, Y0, vals=[s.astype(np.float64) for s in stored])
+    exact64 = oracle_of(orc, probs, X, 1.0, 0.0, Y0)
+    for i in range(len(probs)):
+        got = Y[i].cpu().numpy()
+        assert rel_err(got, exact_stored[i]) <= 2e-6
+        assert rel_err(got, exact64[i]) <= (1e-6 if precision == "fp32" else 6e-3)
+
+
+def test_batch_rejects_bad_input(pls, gen):
+    rp, ci, v = gen.banded(100, 32, 5, 10, 5)
+    A = pls.SpMat.csr(100, 100, rp, ci, v)
+    S = pls.SpMat.csr(100, 100, rp, ci, v.astype(np.float32))
+    with pytest.raises(pls.PerflibsError):
+        pls.Batch([A, S], [1, 1])
+    with pytest.raises(pls.PerflibsError):
+        pls.Batch([A], [3])
+    with pytest.raises(pls.PerflibsError):
+        pls.Batch([A], [1], precision=2)
+    Bt = pls.Batch([A], [1])
+    with pytest.raises(pls.PerflibsError):
+        Bt.exec(1.0, 0.0)  # operands not set
