@@ -124,6 +124,11 @@ perflibs_status_t perflibs_amd_set_selector_model(const float *params,
                                                   const float *feat_mean,
                                                   const float *feat_std);
 perflibs_status_t perflibs_amd_clear_selector_model(void);
+/* Load a model file written by the training driver (plsparse.selector).  The
+ * library loads <dir of libplsparse_amd.so>/../selector/selector_mlp.txt (or the
+ * file named by the environment variable PERFLIBS_AMD_SELECTOR_MODEL) by
+ * itself at the first optimize() when no model was set or cleared. */
+perflibs_status_t perflibs_amd_load_selector_model(const char *path);
 
 /* MLP training/inference primitives on device buffers (FP32), used by the
  * training driver and by the gradient tests.  n = number of samples.

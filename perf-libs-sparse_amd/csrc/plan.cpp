@@ -135,8 +135,8 @@ bool variant_applicable(const perflibs_spmat_top_t *A, const Operand &o, int v) 
 
 bool variant_candidate(const perflibs_spmat_top_t *A, const Operand &o, int v) {
   if (!variant_applicable(A, o, v)) return false;
-  if (v == PERFLIBS_AMD_BSR_B4) return o.fill4 >= 0.25f;
-  if (v == PERFLIBS_AMD_BSR_B8) return o.fill8 >= 0.25f;
+  if (v == PERFLIBS_AMD_BSR_B4) return o.fill4 >= 0.75f;
+  if (v == PERFLIBS_AMD_BSR_B8) return o.fill8 >= 0.75f;
   return true;
 }
 

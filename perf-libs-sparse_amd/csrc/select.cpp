@@ -9,9 +9,16 @@
 //                log(time / best time) per variant, argmin wins;
 //   HEURISTIC  — fixed rules on the same features (the no-network baseline);
 //   FIXED_CSR  — always the CSR-vector kernel.
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
 
 #include "plsparse.hpp"
 
@@ -25,10 +32,69 @@ constexpr size_t kNumParams = size_t(kF) * kH + kH + size_t(kH) * kH + kH + size
 
 struct Model {
   bool loaded = false;
+  bool default_tried = false;  // the shipped model file was looked for
   DevBuf params;
   float mean[kF] = {0}, stdv[kF] = {0};
 };
 Model g_model;
+
+// Model file (text): "plsparse-selector-mlp 1", "features F hidden H outputs K",
+// then lines "mean ...", "std ..." and "params ..." of whitespace-separated
+// floats (the flat parameter layout of perflibs_amd_set_selector_model).
+bool parse_model(const std::string &text, std::vector<float> &params, std::vector<float> &mean,
+                 std::vector<float> &stdv, std::string &why) {
+  std::istringstream in(text);
+  std::string line, magic;
+  int version = 0;
+  if (!std::getline(in, line)) return why = "empty model file", false;
+  std::istringstream h(line);
+  h >> magic >> version;
+  if (magic != "plsparse-selector-mlp" || version != 1) return why = "not a selector model file", false;
+  int f = 0, hd = 0, k = 0;
+  std::string a, b, c;
+  if (!std::getline(in, line)) return why = "truncated model file", false;
+  std::istringstream d(line);
+  d >> a >> f >> b >> hd >> c >> k;
+  if (a != "features" || b != "hidden" || c != "outputs" || f != kF || hd != kH || k != kK)
+    return why = "model shape differs from this library's 12 -> 64 -> 64 -> K network", false;
+  auto read_row = [&](const char *key, std::vector<float> &v, size_t n) -> bool {
+    if (!std::getline(in, line)) return false;
+    std::istringstream r(line);
+    std::string name;
+    r >> name;
+    if (name != key) return false;
+    v.clear();
+    float x;
+    while (r >> x) v.push_back(x);
+    return v.size() == n;
+  };
+  if (!read_row("mean", mean, kF) || !read_row("std", stdv, kF) || !read_row("params", params, kNumParams))
+    return why = "model file rows are missing or have the wrong length", false;
+  return true;
+}
+
+// The model shipped beside the library: <dir of libplsparse_amd.so>/../selector/selector_mlp.txt,
+// or the file named by PERFLIBS_AMD_SELECTOR_MODEL.
+std::string default_model_path() {
+  if (const char *env = std::getenv("PERFLIBS_AMD_SELECTOR_MODEL")) return env;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void *>(&perflibs_amd_load_selector_model), &info) == 0 || !info.dli_fname)
+    return "";
+  std::string so = info.dli_fname;
+  const size_t slash = so.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+  return dir + "/../selector/selector_mlp.txt";
+}
+
+void load_default_model() {
+  if (g_model.loaded || g_model.default_tried) return;
+  g_model.default_tried = true;
+  const std::string path = default_model_path();
+  if (path.empty()) return;
+  std::ifstream f(path);
+  if (!f) return;  // no shipped model: the heuristic stays the default
+  (void)perflibs_amd_load_selector_model(path.c_str());
+}
 }  // namespace
 
 perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, float *feat) {
@@ -155,6 +221,7 @@ int pick_variant(perflibs_spmat_top_t *A, const Operand &o) {
   if (A->forced_variant >= 0)
     return variant_applicable(A, o, A->forced_variant) ? A->forced_variant : PERFLIBS_AMD_CSR_VECTOR;
   int mode = A->select_mode;
+  load_default_model();
   if (mode < 0) mode = g_model.loaded ? PERFLIBS_AMD_SELECT_LEARNED : PERFLIBS_AMD_SELECT_HEURISTIC;
   if (mode == PERFLIBS_AMD_SELECT_FIXED_CSR) return PERFLIBS_AMD_CSR_VECTOR;
   if (mode == PERFLIBS_AMD_SELECT_EXHAUSTIVE) {
@@ -189,7 +256,27 @@ perflibs_status_t perflibs_amd_set_selector_model(const float *params, const flo
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_load_selector_model(const char *path) {
+  if (!path) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  std::ifstream f(path);
+  if (!f) {
+    pls::set_last_error(std::string("cannot read selector model ") + path);
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  }
+  std::stringstream ss;
+  ss << f.rdbuf();
+  std::vector<float> params, mean, stdv;
+  std::string why;
+  if (!pls::parse_model(ss.str(), params, mean, stdv, why)) {
+    pls::set_last_error(std::string("selector model ") + path + ": " + why);
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  }
+  pls::g_model.default_tried = true;
+  return perflibs_amd_set_selector_model(params.data(), mean.data(), stdv.data());
+}
+
 perflibs_status_t perflibs_amd_clear_selector_model(void) {
+  pls::g_model.default_tried = true;  // cleared on purpose: do not reload the shipped file
   pls::g_model.loaded = false;
   pls::g_model.params.reset();
   return PERFLIBS_STATUS_SUCCESS;

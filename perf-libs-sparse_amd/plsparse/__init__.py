@@ -101,6 +101,7 @@ _SIGS = {
     "perflibs_amd_mlp_num_params": (C.c_size_t, []),
     "perflibs_amd_set_selector_model": (_I, [_P, _P, _P]),
     "perflibs_amd_clear_selector_model": (_I, []),
+    "perflibs_amd_load_selector_model": (_I, [C.c_char_p]),
     "perflibs_amd_mlp_forward": (_I, [_P, _P, C.c_int, _P, _P, _P]),
     "perflibs_amd_mlp_backward": (_I, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_mlp_adam": (_I, [_P, _P, _P, _P, C.c_int, _F, _F, _F, _F]),

@@ -162,23 +162,26 @@ def block_random(mb: int, b: int, k: int, w: int, seed: int, nb: int | None = No
 
 
 def bsr_to_csr_rowmajor(brp, bci, bval, b: int):
-    """Expand row-major BSR (0-based) to CSR keeping every stored value."""
+    """Expand row-major BSR (0-based) to CSR keeping every stored value: row
+    rb*b + i holds, block by block, the b values of row i of each block."""
+    brp = np.asarray(brp, np.int64)
+    bci = np.asarray(bci, np.int64)
     mb = len(brp) - 1
-    bb = b * b
-    counts = np.diff(brp).astype(np.int64) * b
+    nnzb = int(brp[-1])
+    counts = np.diff(brp)
     rp = np.zeros(mb * b + 1, np.int64)
-    rp[1:] = np.cumsum(np.repeat(counts, b))
-    nnz = int(rp[-1])
+    rp[1:] = np.cumsum(np.repeat(counts * b, b))
+    rb = np.repeat(np.arange(mb), counts)                # block row of every block
+    k_in_row = np.arange(nnzb) - brp[rb]                 # position of the block in its block row
+    i = np.arange(b)[None, :, None]
+    j = np.arange(b)[None, None, :]
+    base = (brp[rb] * b * b + k_in_row * b)[:, None, None]
+    idx = base + i * (counts[rb] * b)[:, None, None] + j  # CSR slot of value (k, i, j)
+    nnz = nnzb * b * b
     col = np.empty(nnz, np.int32)
     val = np.empty(nnz, np.float64)
-    blocks = bval.reshape(-1, b, b)
-    for rb in range(mb):
-        k0, k1 = brp[rb], brp[rb + 1]
-        cols = (bci[k0:k1, None] * b + np.arange(b)[None, :]).reshape(-1)
-        for i in range(b):
-            r = rb * b + i
-            col[rp[r]:rp[r + 1]] = cols
-            val[rp[r]:rp[r + 1]] = blocks[k0:k1, i, :].reshape(-1)
+    col[idx.reshape(-1)] = np.broadcast_to(bci[:, None, None] * b + j, idx.shape).reshape(-1)
+    val[idx.reshape(-1)] = np.asarray(bval, np.float64)[:nnz]
     return rp.astype(np.int32), col, val
 
 

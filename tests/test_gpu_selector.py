@@ -138,3 +138,29 @@ def test_learned_mode_uses_model(pls, gen):
         assert A.variant() == 4
     finally:
         L.perflibs_amd_clear_selector_model()
+
+
+def test_shipped_model_picks_like_numpy(pls, gen, tmp_path):
+    from pathlib import Path
+
+    from plsparse import selector as S
+
+    path = Path(S.DEFAULT_MODEL)
+    L = pls.lib()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("plsparse-selector-mlp 1\nfeatures 12 hidden 32 outputs 13\n")
+    assert L.perflibs_amd_load_selector_model(str(bad).encode()) == pls.STATUS_INPUT_PARAMETER_ERROR
+    pls.check("load", L.perflibs_amd_load_selector_model(str(path).encode()))
+    params, mean, std = S.load_model(path)
+    for seed, (m, kind) in enumerate(((200_000, "powerlaw"), (90_000, "banded"), (40_000, "stencil"))):
+        mm, n, rp, ci, v, _ = S.sample_matrix(kind, 900 + seed, rows=(m, m + 1))
+        A = pls.SpMat.csr(mm, n, rp, ci, v)
+        A.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
+        f = A.features()
+        A.optimize()  # default mode: learned, since a model is loaded
+        _, _, pred = mlp.forward(params, ((f - mean) / std)[None, :], K)
+        cand = np.ones(K, bool)
+        cand[10] = f[9] >= 0.75 and mm % 4 == 0
+        cand[11] = f[10] >= 0.75 and mm % 8 == 0
+        want = int(np.argmin(np.where(cand, pred[0], np.inf)))
+        assert A.variant() == want, (kind, pls.VARIANTS[A.variant()], pls.VARIANTS[want])
