@@ -140,6 +140,79 @@ def cpu_baseline(n, rp, ci, v, x, y_gpu) -> dict:
     }
 
 
+def batch_bench(args) -> int:
+    """The batched configuration: 4096 banded problems (rows log-uniform in
+    [256, 65536], bandwidth 32-512, 5-40 non-zeros per row, k in {1, 4, 16}) as
+    one persistent launch replayed from a hipGraph, for FP64, FP32 and BF16
+    value storage.  GB/s = sum of every problem's algorithmic bytes / replay
+    time; the FP32 and BF16 results are compared with the FP64 run's (the FP64
+    batch itself is checked against the compensated oracle by the tests)."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from plsparse import workloads as W
+
+    P.set_stream(torch.cuda.current_stream().cuda_stream)
+    t0 = time.perf_counter()
+    probs = W.banded_batch(args.batch, 0x5EED0004)
+    nnz = sum(int(p[1][-1]) for p in probs)
+    log(f"{len(probs)} problems, {sum(p[0] for p in probs)} rows, nnz={nnz} in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(0x5EED1004)
+    X64 = [torch.from_numpy(rng.uniform(-1, 1, (m, k))).cuda() for (m, _, _, _, k) in probs]
+    results, y64 = {}, None
+    for prec in ("fp64", "fp32", "bf16"):
+        dt = np.float64 if prec == "fp64" else np.float32
+        vals = [p[3] if prec == "fp64" else p[3].astype(np.float32) for p in probs]
+        mats = [P.SpMat.csr(m, m, rp, ci, vals[i], flags=P.CREATE_NOCOPY)
+                for i, (m, rp, ci, _, _) in enumerate(probs)]
+        B = P.Batch(mats, [p[4] for p in probs], P.VALUES_BF16 if prec == "bf16" else P.VALUES_NATIVE)
+        X = X64 if prec == "fp64" else [x.float() for x in X64]
+        Y = [torch.empty((m, k), dtype=X[0].dtype, device="cuda") for (m, _, _, _, k) in probs]
+        B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+        t0 = time.perf_counter()
+        B.optimize(autotune=True)
+        tune_s = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            B.exec(1.0, 0.0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            B.exec(1.0, 0.0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        sv = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
+        sx = 8 if prec == "fp64" else 4
+        nbytes = sum(int(rp[-1]) * (sv + 4) + (m + 1) * 4 + m * k * sx * 2 for (m, rp, _, _, k) in probs)
+        info = B.info()
+        entry = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1), "algorithmic_bytes": nbytes,
+                 "autotune_s": round(tune_s, 2), "tile_nnz": info["tile_nnz"], "tiles": info["ntiles"],
+                 "lanes_per_row": info["lanes"], "cols_per_lane": info["cpl"]}
+        ys = np.concatenate([y.double().cpu().numpy().ravel() for y in Y])
+        if prec == "fp64":
+            y64 = ys
+        else:
+            entry["rel_err_vs_fp64"] = float(np.linalg.norm(ys - y64) / np.linalg.norm(y64))
+        results[prec] = entry
+        log(f"{prec}: {entry}")
+        B.destroy()
+        del mats, Y
+        torch.cuda.empty_cache()
+    out = {
+        "metric": METRIC, "value": results["fp64"]["GB/s"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["fp64"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched_banded_4096", "problems": len(probs), "nnz": nnz,
+                   "rhs": [1, 4, 16], "seed": 0x5EED0004},
+        "precisions": results,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,11 +222,16 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-variant exhaustive sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--rows", type=int, default=N_ROWS)
+    ap.add_argument("--workload", choices=("spmv", "batch"), default="spmv",
+                    help="spmv: the headline SpMV line (default); batch: the batched configuration")
+    ap.add_argument("--batch", type=int, default=4096, help="problems of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU is not part of this build",
               file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return batch_bench(args)
 
     import numpy as np
     import torch
