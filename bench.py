@@ -213,6 +213,72 @@ def batch_bench(args) -> int:
     return 0
 
 
+def spmm_bench(args) -> int:
+    """CSR x dense SpMM: the 27-point stencil on 128^3 (n = 2,097,152) times a
+    row-major B of k in {8, 32, 128} columns, FP64 and FP32, and the BSR
+    operator of the same shape (27-point block pattern of dense N(0,1) 4x4 or
+    8x8 blocks) in FP64.  Device-resident B and C, beta = 0; GB/s counts
+    values + indices + B + C once (BSR: block values + block indices)."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from plsparse import workloads as W
+
+    P.set_stream(torch.cuda.current_stream().cuda_stream)
+    rp, ci, v = W.stencil27(128)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    log(f"27-point stencil n={n} nnz={nnz}")
+    lines = {}
+
+    def run(tag, A, k, tdt, nbytes):
+        Bd = torch.rand((n, k), dtype=tdt, device="cuda") * 2 - 1
+        Cd = torch.empty((n, k), dtype=tdt, device="cuda")
+        for _ in range(args.warmup):
+            A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        lines[tag] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
+                      "GFLOP/s": None, "algorithmic_bytes": nbytes}
+        log(f"{tag}: {ms:.3f} ms {nbytes / ms / 1e6:.1f} GB/s")
+        del Bd, Cd
+
+    for prec, dt, tdt, sv in (("fp64", np.float64, torch.float64, 8), ("fp32", np.float32, torch.float32, 4)):
+        A = P.SpMat.csr(n, n, rp, ci, v.astype(dt), flags=P.CREATE_NOCOPY)
+        for k in (8, 32, 128):
+            nbytes = nnz * (sv + 4) + (n + 1) * 4 + n * k * sv * 2
+            run(f"csr_{prec}_k{k}", A, k, tdt, nbytes)
+            lines[f"csr_{prec}_k{k}"]["GFLOP/s"] = round(2 * nnz * k / lines[f"csr_{prec}_k{k}"]["ms"] / 1e6, 1)
+        A.destroy()
+    for b, grid in ((4, (128, 64, 64)), (8, (64, 64, 64))):
+        brp, bci, bv = W.block_stencil(*grid, b, 0x5EED0003)
+        nnzb = int(brp[-1])
+        A = P.SpMat.bsr(P.ROW_MAJOR, n, n, b, brp, bci, bv, flags=P.CREATE_NOCOPY)
+        for k in (8, 32):
+            nbytes = nnzb * b * b * 8 + nnzb * 4 + (n // b + 1) * 4 + n * k * 8 * 2
+            run(f"bsr{b}_fp64_k{k}", A, k, torch.float64, nbytes)
+            lines[f"bsr{b}_fp64_k{k}"]["GFLOP/s"] = round(2 * nnzb * b * b * k / lines[f"bsr{b}_fp64_k{k}"]["ms"] / 1e6, 1)
+        A.destroy()
+        del brp, bci, bv
+    best = lines["csr_fp64_k32"]
+    out = {
+        "metric": METRIC, "value": best["GB/s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "spmm_stencil27_128", "rows": n, "nnz": nnz, "value_line": "csr_fp64_k32"},
+        "lines": lines,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,8 +288,9 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-variant exhaustive sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--rows", type=int, default=N_ROWS)
-    ap.add_argument("--workload", choices=("spmv", "batch"), default="spmv",
-                    help="spmv: the headline SpMV line (default); batch: the batched configuration")
+    ap.add_argument("--workload", choices=("spmv", "batch", "spmm"), default="spmv",
+                    help="spmv: the headline SpMV line (default); batch: the batched configuration; "
+                         "spmm: CSR/BSR x dense on the 27-point stencil")
     ap.add_argument("--batch", type=int, default=4096, help="problems of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -232,6 +299,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "spmm":
+        return spmm_bench(args)
 
     import numpy as np
     import torch

@@ -30,13 +30,31 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rm(
 #pragma unroll
   for (int q = 0; q < CPL; ++q) acc[q] = 0;
   const int32_t p1 = rp[row + 1];
-  for (int32_t p = rp[row]; p < p1; ++p) {
-    const A a = alpha * A(ldv(val, p));
-    const X *b = B + int64_t(ci[p]) * ldb;
+  // U non-zeros per step: their column/value loads issue together, then the
+  // U rows of B, so a row does not wait for one B row at a time (the last
+  // step clamps to the row's last non-zero and masks the repeats).
+  constexpr int U = CPL >= 4 ? 4 : 8;
+  for (int32_t p = rp[row]; p < p1; p += U) {
+    int32_t cc[U];
+    A a[U];
 #pragma unroll
-    for (int q = 0; q < CPL; ++q) {
-      const int c = lane + q * G;
-      if (c < k) acc[q] += a * A(b[c]);
+    for (int u = 0; u < U; ++u) {
+      const int32_t pp = min(p + u, p1 - 1);
+      cc[u] = ldi(ci, pp);
+      a[u] = alpha * A(ldv(val, pp));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const X *b = B + int64_t(cc[u]) * ldb;
+      const bool ok = p + u < p1;
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const int c = lane + q * G;
+        if (c < k) {
+          const A v = a[u] * A(b[c]);
+          acc[q] += ok ? v : A(0);
+        }
+      }
     }
   }
   X *crow = Cm + row * ldc;
