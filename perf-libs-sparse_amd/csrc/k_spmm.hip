@@ -4,7 +4,7 @@
 // (spmm_rowwise_csr_blocked_m_kernel, src/compressed_sparse_rows.cpp:1699-1745,
 // and the C zeroing of src/matmul.cpp:596-605): beta == 0 never reads C.
 //
-// Row-major fast path: a group of G lanes owns one row of C (G = 8 for k = 8,
+// Row-major path: a group of G lanes owns one row of C (G = 8 for k = 8,
 // so a wave covers 8 rows; G = 32 for k = 32; G = 64 with 2 columns per lane for
 // k = 128).  For each non-zero a_p of the row the group reads the k-element row
 // B(col_p, :) as one contiguous access and accumulates alpha*a_p*B in
@@ -69,6 +69,106 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rm(
   }
 }
 
+// Row-block variant (dispatched for k <= 16): one workgroup per row block of the
+// CSR plan's nnz-balanced block table (at most kSpmmStage non-zeros and
+// kSpmmRows rows, a longer row alone in its block).  The block's column
+// indices, alpha-scaled values and row offsets are staged into LDS with
+// coalesced loads first, so the per-row loop of each lane group only waits for
+// the B rows it gathers (U rows in flight per lane), never for the matrix
+// stream.  Blocks too long to stage read the matrix from HBM directly.
+constexpr int kSpmmStage = 2048;  // = the CSR plan's row-block non-zeros (kAdaNnz)
+constexpr int kSpmmRows = 1024;   // = the CSR plan's row-block rows
+
+template <int G, int CPL, class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_spmm_blk(
+    const int32_t *__restrict__ blocks, int32_t k, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const V *__restrict__ val, const X *__restrict__ B,
+    int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  __shared__ int32_t s_ci[kSpmmStage];
+  __shared__ A s_a[kSpmmStage];
+  __shared__ int32_t s_rp[kSpmmRows + 1];
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int32_t r0 = blocks[blk], r1 = blocks[blk + 1];
+  const int32_t e0 = rp[r0], e1 = rp[r1];
+  const int32_t nnz = e1 - e0, nrows = r1 - r0;
+  const bool staged = nnz <= kSpmmStage && nrows <= kSpmmRows;
+  if (staged) {
+    constexpr int PER = kSpmmStage / kBlock;
+    int32_t c[PER];
+    A a[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int32_t q = min(tid + u * kBlock, max(nnz - 1, 0));
+      c[u] = nnz > 0 ? ldi(ci, e0 + q) : 0;
+      a[u] = nnz > 0 ? alpha * A(ldv(val, e0 + q)) : A(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int32_t q = tid + u * kBlock;
+      if (q < nnz) {
+        s_ci[q] = c[u];
+        s_a[q] = a[u];
+      }
+    }
+    for (int32_t q = tid; q <= nrows; q += kBlock) s_rp[q] = rp[r0 + q] - e0;
+  }
+  __syncthreads();
+  // S lane groups share a row when the block has few (long) rows, so every
+  // group has work: S = the largest power of two with nrows * S groups fitting
+  // the workgroup and G * S lanes fitting a wave; the S partial sums of a row
+  // are folded with xor shuffles inside its W = G * S lanes.
+  int S = 1;
+  while (S * 2 * G <= 64 && nrows * S * 2 <= kBlock / G) S *= 2;
+  const int W = G * S;
+  const int lane = tid & (G - 1), sub = (tid & (W - 1)) / G;
+  constexpr int U = CPL >= 4 ? 8 : 16;  // B rows in flight per lane
+  for (int32_t rr = tid / W; rr < nrows; rr += kBlock / W) {
+    const int32_t p0 = staged ? s_rp[rr] : rp[r0 + rr] - e0;
+    const int32_t p1 = staged ? s_rp[rr + 1] : rp[r0 + rr + 1] - e0;
+    A acc[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) acc[q] = 0;
+    for (int32_t p = p0 + sub; p < p1; p += U * S) {
+      int32_t cc[U];
+      A a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t pp = min(p + u * S, p1 - 1);
+        cc[u] = staged ? s_ci[pp] : ldi(ci, e0 + pp);
+        a[u] = staged ? s_a[pp] : alpha * A(ldv(val, e0 + pp));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const X *b = B + int64_t(cc[u]) * ldb;
+        const bool ok = p + u * S < p1;
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+          const int col = lane + q * G;
+          if (col < k) {
+            const A v = a[u] * A(b[col]);
+            acc[q] += ok ? v : A(0);
+          }
+        }
+      }
+    }
+    for (int o = G; o < W; o <<= 1)  // all W lanes of the row take this path together
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) acc[q] += __shfl_xor(acc[q], o);
+    X *crow = Cm + int64_t(r0 + rr) * ldc;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int col = lane + q * G;
+      if (sub == 0 && col < k) {
+        A r = acc[q];
+        if (BETA) r += beta * A(crow[col]);
+        crow[col] = X(r);
+      }
+    }
+  }
+}
+
 template <class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_strided(
     int32_t m, int32_t k, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -106,6 +206,20 @@ namespace {
 template <int G, int CPL, class V, class X>
 void launch_rm(const pls::DevCsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
                double beta, void *C, int64_t ldc, hipStream_t s) {
+  // Narrow B (k <= 16): the row-block kernel over the plan's block table.
+  // Wider B rows keep one row per lane group and far more rows in flight
+  // (measured faster for k = 32 and 128 on the 27-point stencil).
+  if (A.nblocks > 0 && G * CPL <= 16) {
+    if (beta != 0.0)
+      k_spmm_blk<G, CPL, V, X, true><<<A.nblocks, kBlock, 0, s>>>(
+          A.blocks.as<int32_t>(), k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
+    else
+      k_spmm_blk<G, CPL, V, X, false><<<A.nblocks, kBlock, 0, s>>>(
+          A.blocks.as<int32_t>(), k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
+    return;
+  }
   const unsigned grid = grid_for(int64_t(A.m) * G);
   if (grid == 0) return;
   if (beta != 0.0)

@@ -153,3 +153,30 @@ def test_stencil27_full_k8(pls, orc, gen):
     pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, ones, 0.0, C1)
     g = C1.export_dense()[:, 0].reshape(128, 128, 128)
     assert np.all(g[1:-1, 1:-1, 1:-1] == 0.0) and np.all(g[0] > 0)
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
+def test_spmm_long_and_short_rows(pls, orc, k):
+    """Row blocks too long to stage (rows of 3000 non-zeros), blocks of few long
+    rows that several lane groups share, and empty rows, through the async path."""
+    import torch
+
+    rng = np.random.default_rng(50 + k)
+    m, n = 3000, 7000
+    lens = rng.integers(0, 40, m)
+    lens[rng.random(m) < 0.1] = 0
+    lens[::97] = 3000
+    lens[1500:1540] = 400
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(l), replace=False)) for l in lens]).astype(np.int32)
+    v = rng.standard_normal(int(rp[-1]))
+    Bv = rng.uniform(-1, 1, (n, k))
+    C0 = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, n, rp, ci, v)
+    Bd = torch.from_numpy(Bv).cuda()
+    for beta in (0.0, 0.5):
+        Cd = torch.from_numpy(C0.copy()).cuda()
+        A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, -1.25, beta)
+        torch.cuda.synchronize()
+        ref = orc.spmm(m, k, rp, ci, v, Bv, -1.25, beta, C0 if beta else None)
+        assert rel_err(Cd.cpu().numpy(), ref) <= TOL64
