@@ -37,7 +37,7 @@ struct perflibs_amd_batch_s {
   double mean_row[kBatchClasses] = {0, 0, 0, 0, 0};
   pls::DevBuf rp, ci, vals, probs_d, tiles_d, xs, ys;
   bool operands = false;
-  int32_t tile_nnz = 4096;
+  int32_t tile_nnz = 2048;
   BatchShape shape{};
   int32_t ntiles = 0, grid = 0;
   int32_t cus = 1, occ = 1;
@@ -81,7 +81,8 @@ perflibs_status_t batch_fail(perflibs_status_t st, const std::string &msg) {
 }
 
 // Row tiles of at most tile_nnz non-zeros (a longer row is a tile of its own)
-// and at most 4096 rows, sorted by estimated bytes moved, largest first.
+// and at most 1024 rows (so a tile of up to 2048 non-zeros is staged in LDS by
+// the kernel), sorted by estimated bytes moved, largest first.
 void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<BatchTile> &tiles) {
   struct Costed {
     double cost;
@@ -97,7 +98,7 @@ void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<Ba
     while (r < pb.m) {
       const int32_t r0 = r;
       int64_t nz = 0;
-      while (r < pb.m && r - r0 < 4096) {
+      while (r < pb.m && r - r0 < 1024) {
         const int64_t len = rp[r + 1] - rp[r];
         if (r > r0 && nz + len > tile_nnz) break;
         nz += len;
@@ -401,7 +402,7 @@ perflibs_status_t perflibs_amd_batch_optimize(perflibs_amd_batch_t B, int autotu
   double best = 0;
   e = time_current(best);
   // 1) tile size
-  const int32_t tile_cands[] = {2048, 4096, 8192, 16384, 65536};
+  const int32_t tile_cands[] = {256, 512, 1024, 2048, 8192};
   int32_t best_tile = B->tile_nnz;
   for (int32_t tn : tile_cands) {
     if (e != hipSuccess || tn == best_tile) continue;

@@ -19,13 +19,27 @@
 
 namespace plk {
 
+// A tile's column indices, alpha-scaled values and row offsets are staged in
+// LDS when they fit (at most kBatchStage non-zeros and kBatchRows rows), so
+// the per-row loops only wait for the X rows they gather; a tile that does not
+// fit (a row longer than kBatchStage) reads the matrix from HBM directly.
+constexpr int kBatchStage = 2048;
+constexpr int kBatchRows = 1024;
+
+template <class A> struct BatchLds {
+  int32_t ci[kBatchStage];
+  A a[kBatchStage];
+  int32_t rp[kBatchRows + 1];
+};
+
 template <int CPL, class V, class X, bool BETA>
 __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProblem &pb, int G,
                                            const int32_t *__restrict__ rp_all,
                                            const int32_t *__restrict__ ci_all,
                                            const V *__restrict__ val_all,
                                            const X *__restrict__ xp, X *__restrict__ yp,
-                                           acc_t<X> alpha, acc_t<X> beta) {
+                                           acc_t<X> alpha, acc_t<X> beta,
+                                           BatchLds<acc_t<X>> &lds) {
   using A = acc_t<X>;
   const int k = pb.k;
   const int kc = k / CPL;  // lanes spanning one row's k columns
@@ -38,27 +52,53 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
   const int sub = lane / kc, cb = (lane % kc) * CPL;
   const int rows_per_pass = kBlock / G;
   const bool products = alpha != A(0);
-  for (int32_t base = tl.r0; base < tl.r1; base += rows_per_pass) {
-    const int32_t row = base + grp;
+  const int32_t e0 = rp[tl.r0];
+  const int32_t nnz = rp[tl.r1] - e0, nrows = tl.r1 - tl.r0;
+  const bool staged = nnz <= kBatchStage && nrows <= kBatchRows;
+  __syncthreads();  // the previous tile is done with the LDS
+  if (staged) {
+    constexpr int PER = kBatchStage / kBlock;
+    int32_t c[PER];
+    A a[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int32_t q = min(tid + u * kBlock, max(nnz - 1, 0));
+      c[u] = nnz > 0 ? ldi(ci, e0 + q) : 0;
+      a[u] = nnz > 0 ? alpha * A(ldv(va, e0 + q)) : A(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int32_t q = tid + u * kBlock;
+      if (q < nnz) {
+        lds.ci[q] = c[u];
+        lds.a[q] = a[u];
+      }
+    }
+    for (int32_t q = tid; q <= nrows; q += kBlock) lds.rp[q] = rp[tl.r0 + q] - e0;
+  }
+  __syncthreads();
+  for (int32_t base = 0; base < nrows; base += rows_per_pass) {
+    const int32_t rr = base + grp;
     A acc[CPL];
 #pragma unroll
     for (int t = 0; t < CPL; ++t) acc[t] = A(0);
-    if (row < tl.r1 && products) {
-      const int32_t e = rp[row + 1];
-      constexpr int U = 4;  // non-zeros in flight per lane
-      for (int32_t j = rp[row] + sub; j < e; j += U * S) {
+    if (rr < nrows && products) {
+      const int32_t p0 = staged ? lds.rp[rr] : rp[tl.r0 + rr] - e0;
+      const int32_t p1 = staged ? lds.rp[rr + 1] : rp[tl.r0 + rr + 1] - e0;
+      constexpr int U = CPL >= 4 ? 4 : 8;  // non-zeros in flight per lane
+      for (int32_t j = p0 + sub; j < p1; j += U * S) {
         int32_t c[U];
         A a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int32_t jj = min(j + u * S, e - 1);
-          c[u] = ldi(ci, jj);
-          a[u] = A(ldv(va, jj));
+          const int32_t jj = min(j + u * S, p1 - 1);
+          c[u] = staged ? lds.ci[jj] : ldi(ci, e0 + jj);
+          a[u] = staged ? lds.a[jj] : alpha * A(ldv(va, e0 + jj));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const X *xr = xp + int64_t(c[u]) * k + cb;
-          const bool ok = j + u * S < e;
+          const bool ok = j + u * S < p1;
 #pragma unroll
           for (int t = 0; t < CPL; ++t) {
             const A p = a[u] * A(xr[t]);
@@ -70,11 +110,11 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
     for (int o = kc; o < G; o <<= 1)
 #pragma unroll
       for (int t = 0; t < CPL; ++t) acc[t] += __shfl_xor(acc[t], o);
-    if (row < tl.r1 && sub == 0) {
-      X *yr = yp + int64_t(row) * k + cb;
+    if (rr < nrows && sub == 0) {
+      X *yr = yp + int64_t(tl.r0 + rr) * k + cb;
 #pragma unroll
       for (int t = 0; t < CPL; ++t) {
-        A r = alpha * acc[t];
+        A r = acc[t];
         if (BETA) r += beta * A(yr[t]);
         yr[t] = X(r);
       }
@@ -88,6 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(
     const int32_t *__restrict__ rp_all, const int32_t *__restrict__ ci_all,
     const V *__restrict__ val_all, const X *const *__restrict__ xs, X *const *__restrict__ ys,
     acc_t<X> alpha, acc_t<X> beta, BatchShape shape) {
+  __shared__ BatchLds<acc_t<X>> lds;
   for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchTile tl = tiles[t];
     const BatchProblem pb = probs[tl.prob];
@@ -96,11 +137,11 @@ __global__ __launch_bounds__(kBlock) void k_batch(
     const int G = shape.lanes[pb.kclass];
     const int cpl = shape.cpl[pb.kclass];
     if (cpl == 1)
-      batch_tile<1, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta);
+      batch_tile<1, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta, lds);
     else if (cpl == 2)
-      batch_tile<2, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta);
+      batch_tile<2, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta, lds);
     else
-      batch_tile<4, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta);
+      batch_tile<4, V, X, BETA>(tl, pb, G, rp_all, ci_all, val_all, xp, yp, alpha, beta, lds);
   }
 }
 
