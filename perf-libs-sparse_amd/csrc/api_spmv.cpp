@@ -118,6 +118,12 @@ perflibs_status_t spmv_optimize_impl(perflibs_spmat_top_t *A) {
     const int v = pick_variant(A, o);
     if (v == 64 && A->fmt == Fmt::Bsr) st = build_native_bsr_plan(A);
     else st = build_plan(A, o, op, v);
+    // a selected (not forced) layout that cannot be built, e.g. out of HBM for
+    // SELL padding, falls back to the plain CSR kernel
+    if (st != PERFLIBS_STATUS_SUCCESS && A->forced_variant < 0 && v != PERFLIBS_AMD_CSR_VECTOR) {
+      (void)hipGetLastError();
+      st = build_plan(A, o, op, PERFLIBS_AMD_CSR_VECTOR);
+    }
   }
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   if (A->plan.sell) {  // the matrix object becomes SELL-C-sigma
