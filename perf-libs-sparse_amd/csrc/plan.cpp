@@ -277,7 +277,11 @@ static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand
   if (e != hipSuccess) return hip_fail(A, e, "device query");
   cus = std::max(cus, 1);
   int32_t pw = A->forced_panel_log2 > 0 ? A->forced_panel_log2 : (dbl ? 16 : 17);
-  while (pw < 30 && ((int64_t(o.n) - 1) >> pw) >= 65536) ++pw;  // at most 65536 panels
+  // at most 65536 panels, and at most 2^25 (tile, panel) segments for up to
+  // cus * 16 tiles, so the segment table stays small for very wide operands
+  while (pw < 30 && (((int64_t(o.n) - 1) >> pw) >= 65536 ||
+                     ((((int64_t(o.n) - 1) >> pw) + 1) * cus * 16 > (int64_t(1) << 25))))
+    ++pw;
   // row field of the packed index: tile rows plus the all-ones padding marker
   const int32_t rmax_idx = int32_t(std::min<int64_t>((int64_t(1) << (32 - pw)) - 64, 1 << 16));
   constexpr size_t kMaxLds = 64 * 1024;  // per workgroup
