@@ -116,6 +116,28 @@ perflibs_spmat_create_csr_d(perflibs_spmat_t *A, perflibs_int_t m,
                             perflibs_int_t n, const perflibs_int_t *row_ptr,
                             const perflibs_int_t *col_indx, const double *vals,
                             perflibs_int_t flags);
+/* CSC and COO ingest (reference include/perflibs_sparse.h:176-200): the
+ * arrays are converted to a CSR object at creation (always a copy, also with
+ * PERFLIBS_SPARSE_CREATE_NOCOPY); CSC keeps the index base of col_ptr[0], COO
+ * is 0-based like the reference's create_coo. */
+perflibs_status_t
+perflibs_spmat_create_csc_s(perflibs_spmat_t *A, perflibs_int_t m,
+                            perflibs_int_t n, const perflibs_int_t *row_indx,
+                            const perflibs_int_t *col_ptr, const float *vals,
+                            perflibs_int_t flags);
+perflibs_status_t
+perflibs_spmat_create_csc_d(perflibs_spmat_t *A, perflibs_int_t m,
+                            perflibs_int_t n, const perflibs_int_t *row_indx,
+                            const perflibs_int_t *col_ptr, const double *vals,
+                            perflibs_int_t flags);
+perflibs_status_t perflibs_spmat_create_coo_s(
+    perflibs_spmat_t *A, perflibs_int_t m, perflibs_int_t n, perflibs_int_t nnz,
+    const perflibs_int_t *row_indx, const perflibs_int_t *col_indx,
+    const float *vals, perflibs_int_t flags);
+perflibs_status_t perflibs_spmat_create_coo_d(
+    perflibs_spmat_t *A, perflibs_int_t m, perflibs_int_t n, perflibs_int_t nnz,
+    const perflibs_int_t *row_indx, const perflibs_int_t *col_indx,
+    const double *vals, perflibs_int_t flags);
 perflibs_status_t perflibs_spmat_create_dense_s(
     perflibs_spmat_t *A, enum perflibs_dense_layout layout, perflibs_int_t m,
     perflibs_int_t n, perflibs_int_t lda, const float *vals,

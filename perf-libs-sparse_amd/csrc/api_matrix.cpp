@@ -9,6 +9,8 @@
 #include <cstring>
 #include <new>
 
+#include <algorithm>
+
 #include "plsparse.hpp"
 
 using pls::DType;
@@ -63,6 +65,104 @@ perflibs_status_t create_csr(perflibs_spmat_t *out, perflibs_int_t m, perflibs_i
     return pls::exec_fail(A, "host allocation failed");
   }
   return PERFLIBS_STATUS_SUCCESS;
+}
+
+// Adopt a CSR the library built (0-based or 1-based by `base`) as the handle's own.
+template <class T>
+perflibs_status_t adopt_csr(perflibs_spmat_top_t *A, perflibs_int_t m, perflibs_int_t n,
+                            int32_t base, std::vector<int32_t> &&rp, std::vector<int32_t> &&ci,
+                            std::vector<T> &&vals) {
+  A->m = m;
+  A->n = n;
+  A->index_base = base;
+  A->nnz = int32_t(ci.size());
+  A->fmt = Fmt::Csr;
+  A->nocopy = false;
+  A->csr.rp_own = std::move(rp);
+  A->csr.ci_own = std::move(ci);
+  if constexpr (std::is_same_v<T, double>) A->csr.vd_own = std::move(vals);
+  else A->csr.vf_own = std::move(vals);
+  A->csr.point_to_own(A->dt);
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+// COO ingest: the checks of fill_initial_data_coo (src/coordinate_list.cpp:131-168)
+// and the order of coo2csr (src/compressed_sparse_rows.cpp:590-640): entries sorted
+// by row, then column (stably, so duplicates keep their input order), index base 0.
+// Indices outside the matrix are rejected (code 5 / 6; the reference does not check).
+template <class T>
+perflibs_status_t create_coo(perflibs_spmat_t *out, perflibs_int_t m, perflibs_int_t n,
+                             perflibs_int_t nnz, const perflibs_int_t *row_indx,
+                             const perflibs_int_t *col_indx, const T *vals) {
+  if (!out) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  auto *A = new_handle(dtype_of<T>());
+  if (!A) return PERFLIBS_STATUS_EXECUTION_FAILURE;
+  *out = A;
+  if (m < 0) return pls::param_fail(A, 2);
+  if (n < 0) return pls::param_fail(A, 3);
+  if (nnz < 0) return pls::param_fail(A, 4);
+  if (nnz > 0 && (!row_indx || !col_indx || !vals)) return pls::param_fail(A, 5);
+  for (perflibs_int_t i = 0; i < nnz; ++i) {
+    if (row_indx[i] < 0 || row_indx[i] >= m) return pls::param_fail(A, 5, "COO row index out of range");
+    if (col_indx[i] < 0 || col_indx[i] >= n) return pls::param_fail(A, 6, "COO column index out of range");
+  }
+  try {
+    std::vector<int64_t> order(static_cast<size_t>(nnz));
+    for (perflibs_int_t i = 0; i < nnz; ++i) order[size_t(i)] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      return row_indx[a] != row_indx[b] ? row_indx[a] < row_indx[b] : col_indx[a] < col_indx[b];
+    });
+    std::vector<int32_t> rp(size_t(m) + 1, 0), ci(static_cast<size_t>(nnz));
+    std::vector<T> v(static_cast<size_t>(nnz));
+    for (perflibs_int_t i = 0; i < nnz; ++i) ++rp[size_t(row_indx[i]) + 1];
+    for (perflibs_int_t r = 0; r < m; ++r) rp[size_t(r) + 1] += rp[size_t(r)];
+    for (size_t q = 0; q < order.size(); ++q) {
+      ci[q] = col_indx[order[q]];
+      v[q] = vals[order[q]];
+    }
+    return adopt_csr<T>(A, m, n, 0, std::move(rp), std::move(ci), std::move(v));
+  } catch (const std::bad_alloc &) {
+    return pls::exec_fail(A, "host allocation failed");
+  }
+}
+
+// CSC ingest: the checks of fill_initial_data_csc (src/compressed_sparse_columns.cpp:
+// 152-193) and csc2csr_struct / csc2csr (src/compressed_sparse_rows.cpp:749-811):
+// rows filled by a scan over the columns in order, index base of col_ptr[0] kept.
+template <class T>
+perflibs_status_t create_csc(perflibs_spmat_t *out, perflibs_int_t m, perflibs_int_t n,
+                             const perflibs_int_t *row_indx, const perflibs_int_t *col_ptr,
+                             const T *vals) {
+  if (!out) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  auto *A = new_handle(dtype_of<T>());
+  if (!A) return PERFLIBS_STATUS_EXECUTION_FAILURE;
+  *out = A;
+  if (m < 0) return pls::param_fail(A, 2);
+  if (n < 0) return pls::param_fail(A, 3);
+  if (!col_ptr || (col_ptr[0] != 0 && col_ptr[0] != 1)) return pls::param_fail(A, 4);
+  const int32_t base = col_ptr[0];
+  const int64_t nnz = int64_t(col_ptr[n]) - base;
+  if (nnz < 0) return pls::param_fail(A, 4);
+  for (int64_t k = 0; k < nnz; ++k)
+    if (row_indx[k] - base < 0 || row_indx[k] - base >= m)
+      return pls::param_fail(A, 5, "CSC row index out of range");
+  try {
+    std::vector<int32_t> rp(size_t(m) + 1, 0), ci(static_cast<size_t>(nnz));
+    std::vector<T> v(static_cast<size_t>(nnz));
+    for (int64_t k = 0; k < nnz; ++k) ++rp[size_t(row_indx[k] - base) + 1];
+    for (perflibs_int_t r = 0; r < m; ++r) rp[size_t(r) + 1] += rp[size_t(r)];
+    std::vector<int32_t> fill(rp.begin(), rp.end() - 1);
+    for (perflibs_int_t j = 0; j < n; ++j)
+      for (int64_t k = col_ptr[j] - base; k < col_ptr[j + 1] - base; ++k) {
+        const int32_t at = fill[size_t(row_indx[k] - base)]++;
+        ci[size_t(at)] = j + base;
+        v[size_t(at)] = vals[k];
+      }
+    for (auto &x : rp) x += base;
+    return adopt_csr<T>(A, m, n, base, std::move(rp), std::move(ci), std::move(v));
+  } catch (const std::bad_alloc &) {
+    return pls::exec_fail(A, "host allocation failed");
+  }
 }
 
 // BSR creation checks and sizes (src/block_sparse_rows.cpp:194-266).
@@ -468,6 +568,36 @@ perflibs_status_t perflibs_spmat_create_csr_d(perflibs_spmat_t *A, perflibs_int_
                                               const perflibs_int_t *col_indx, const double *vals,
                                               perflibs_int_t flags) {
   return create_csr<double>(A, m, n, row_ptr, col_indx, vals, flags);
+}
+perflibs_status_t perflibs_spmat_create_csc_s(perflibs_spmat_t *A, perflibs_int_t m,
+                                              perflibs_int_t n, const perflibs_int_t *row_indx,
+                                              const perflibs_int_t *col_ptr, const float *vals,
+                                              perflibs_int_t flags) {
+  (void)flags;
+  return create_csc<float>(A, m, n, row_indx, col_ptr, vals);
+}
+perflibs_status_t perflibs_spmat_create_csc_d(perflibs_spmat_t *A, perflibs_int_t m,
+                                              perflibs_int_t n, const perflibs_int_t *row_indx,
+                                              const perflibs_int_t *col_ptr, const double *vals,
+                                              perflibs_int_t flags) {
+  (void)flags;
+  return create_csc<double>(A, m, n, row_indx, col_ptr, vals);
+}
+perflibs_status_t perflibs_spmat_create_coo_s(perflibs_spmat_t *A, perflibs_int_t m,
+                                              perflibs_int_t n, perflibs_int_t nnz,
+                                              const perflibs_int_t *row_indx,
+                                              const perflibs_int_t *col_indx, const float *vals,
+                                              perflibs_int_t flags) {
+  (void)flags;
+  return create_coo<float>(A, m, n, nnz, row_indx, col_indx, vals);
+}
+perflibs_status_t perflibs_spmat_create_coo_d(perflibs_spmat_t *A, perflibs_int_t m,
+                                              perflibs_int_t n, perflibs_int_t nnz,
+                                              const perflibs_int_t *row_indx,
+                                              const perflibs_int_t *col_indx, const double *vals,
+                                              perflibs_int_t flags) {
+  (void)flags;
+  return create_coo<double>(A, m, n, nnz, row_indx, col_indx, vals);
 }
 perflibs_status_t perflibs_spmat_create_dense_s(perflibs_spmat_t *A,
                                                 enum perflibs_dense_layout layout,

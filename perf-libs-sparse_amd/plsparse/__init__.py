@@ -61,6 +61,10 @@ _F = C.c_float
 _SIGS = {
     "perflibs_spmat_create_csr_s": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
     "perflibs_spmat_create_csr_d": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_csc_s": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_csc_d": (_I, [_PP, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_coo_s": (_I, [_PP, _I, _I, _I, _P, _P, _P, _I]),
+    "perflibs_spmat_create_coo_d": (_I, [_PP, _I, _I, _I, _P, _P, _P, _I]),
     "perflibs_spmat_create_dense_s": (_I, [_PP, C.c_int, _I, _I, _I, _P, _I]),
     "perflibs_spmat_create_dense_d": (_I, [_PP, C.c_int, _I, _I, _I, _P, _I]),
     "perflibs_spmat_create_bsr_s": (_I, [_PP, C.c_int, _I, _I, _I, _P, _P, _P, _I]),
@@ -187,6 +191,34 @@ class SpMat:
         fn = f"perflibs_spmat_create_csr_{cls._sfx(vals.dtype)}"
         st = getattr(lib(), fn)(C.byref(h), m, n, _ptr(rp), _ptr(ci), _ptr(vals), flags)
         obj = cls(h.value, vals.dtype, keep=(rp, ci, vals))
+        if raise_on_error:
+            check(fn, st)
+            return obj
+        return obj, st
+
+    @classmethod
+    def csc(cls, m, n, row_indx, col_ptr, vals, flags=0, raise_on_error=True):
+        vals = np.ascontiguousarray(vals)
+        ri = np.ascontiguousarray(row_indx, dtype=np.int32)
+        cp = np.ascontiguousarray(col_ptr, dtype=np.int32)
+        h = C.c_void_p()
+        fn = f"perflibs_spmat_create_csc_{cls._sfx(vals.dtype)}"
+        st = getattr(lib(), fn)(C.byref(h), m, n, _ptr(ri), _ptr(cp), _ptr(vals), flags)
+        obj = cls(h.value, vals.dtype)
+        if raise_on_error:
+            check(fn, st)
+            return obj
+        return obj, st
+
+    @classmethod
+    def coo(cls, m, n, row_indx, col_indx, vals, flags=0, raise_on_error=True):
+        vals = np.ascontiguousarray(vals)
+        ri = np.ascontiguousarray(row_indx, dtype=np.int32)
+        ci = np.ascontiguousarray(col_indx, dtype=np.int32)
+        h = C.c_void_p()
+        fn = f"perflibs_spmat_create_coo_{cls._sfx(vals.dtype)}"
+        st = getattr(lib(), fn)(C.byref(h), m, n, len(ri), _ptr(ri), _ptr(ci), _ptr(vals), flags)
+        obj = cls(h.value, vals.dtype)
         if raise_on_error:
             check(fn, st)
             return obj

@@ -157,3 +157,48 @@ def test_spmm_parameter_checks(pls):
     C = pls.SpMat.null(2, 6)
     assert pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, Bf, 0.0, C, raise_on_error=False) == 1
     assert pls.spmm_exec(999, pls.NOTRANS, 1.0, A, B, 0.0, C, raise_on_error=False) == 1
+
+
+def test_coo_ingest_sorted_like_coo2csr(pls):
+    """COO entries become a 0-based CSR sorted by (row, column) like the
+    reference's coo2csr (src/compressed_sparse_rows.cpp:590-640); duplicates stay."""
+    rows = [2, 0, 2, 1, 0, 2]
+    cols = [1, 3, 0, 2, 0, 1]
+    vals = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    A = pls.SpMat.coo(3, 4, rows, cols, vals)
+    assert A.query() == (0, 3, 4, 6)
+    m, n, rp, ci, v = A.export_csr(0)
+    assert (m, n) == (3, 4)
+    assert rp.tolist() == [0, 2, 3, 6]
+    assert ci.tolist() == [0, 3, 2, 0, 1, 1]
+    assert v.tolist() == [5.0, 2.0, 4.0, 3.0, 1.0, 6.0]  # duplicate (2,1) keeps input order
+    S = pls.SpMat.coo(3, 4, rows, cols, vals.astype(np.float32))
+    assert S.export_csr(0)[4].dtype == np.float32
+    assert pls.SpMat.coo(-1, 4, rows, cols, vals, raise_on_error=False)[1] == 1
+    assert pls.SpMat.coo(3, -4, rows, cols, vals, raise_on_error=False)[1] == 1
+    assert pls.SpMat.coo(3, 4, [3], [0], [1.0], raise_on_error=False)[1] == 1
+    assert pls.SpMat.coo(3, 4, [0], [4], [1.0], raise_on_error=False)[1] == 1
+    E = pls.SpMat.coo(2, 2, [], [], np.zeros(0))
+    assert E.export_csr(0)[2].tolist() == [0, 0, 0]
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_csc_ingest_matches_transpose(pls, base):
+    """CSC arrays of A are the CSR arrays of A^T; the ingest rebuilds A's CSR
+    (csc2csr, src/compressed_sparse_rows.cpp:749-811) and keeps the index base."""
+    rng = np.random.default_rng(9 + base)
+    m, n = 7, 5
+    D = np.where(rng.random((m, n)) < 0.4, rng.standard_normal((m, n)), 0.0)
+    col_ptr = np.concatenate([[0], np.cumsum((D != 0).sum(axis=0))]) + base
+    row_indx = np.concatenate([np.nonzero(D[:, j])[0] for j in range(n)]) + base
+    vals = np.concatenate([D[np.nonzero(D[:, j])[0], j] for j in range(n)])
+    A = pls.SpMat.csc(m, n, row_indx, col_ptr, vals)
+    assert A.query() == (base, m, n, int((D != 0).sum()))
+    _, _, rp, ci, v = A.export_csr(base)
+    E = np.zeros((m, n))
+    for i in range(m):
+        assert np.all(np.diff(ci[rp[i] - base:rp[i + 1] - base]) > 0)
+        E[i, ci[rp[i] - base:rp[i + 1] - base] - base] = v[rp[i] - base:rp[i + 1] - base]
+    assert np.array_equal(E, D)
+    assert pls.SpMat.csc(m, n, row_indx, col_ptr + 2, vals, raise_on_error=False)[1] == 1
+    assert pls.SpMat.csc(-1, n, row_indx, col_ptr, vals, raise_on_error=False)[1] == 1

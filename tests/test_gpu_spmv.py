@@ -379,3 +379,28 @@ def test_powerlaw_4m_full(pls, orc, gen):
             y2 = np.zeros(n)
             A.spmv(2 * x, y2)
             assert np.array_equal(y2, 2 * y)
+
+
+def test_coo_and_csc_objects(pls, orc):
+    """SpMV on objects created from COO and CSC arrays (converted to CSR at creation)."""
+    rng = np.random.default_rng(123)
+    m, n = 3000, 2500
+    rp, ci, v = powerlawish(rng, m, n, mean=9.0)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    perm = rng.permutation(len(v))
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    ref = orc.spmv(m, rp, ci, v, x, 0.5, 2.0, y0)
+    A = pls.SpMat.coo(m, n, rows[perm], ci[perm], v[perm])
+    A.optimize()
+    y = y0.copy()
+    A.spmv(x, y, 0.5, 2.0)
+    assert rel_err(y, ref) <= TOL64
+    # CSC of A = CSR of A^T
+    order = np.lexsort((rows, ci))
+    col_ptr = np.concatenate([[0], np.cumsum(np.bincount(ci, minlength=n))]).astype(np.int32) + 1
+    Bc = pls.SpMat.csc(m, n, rows[order] + 1, col_ptr, v[order])
+    Bc.optimize()
+    y = y0.copy()
+    Bc.spmv(x, y, 0.5, 2.0)
+    assert rel_err(y, ref) <= TOL64
