@@ -12,32 +12,6 @@ namespace plk {
 constexpr int kStats = 8;  // sum len, sum len^2, max len, empty rows,
                            // sum spread, max |col-row|, 4x4 blocks, 8x8 blocks
 
-// Distinct b-wide column blocks touched by rows [rb*b, rb*b+b) (rows with
-// ascending columns; unsorted rows over-count).
-template <int B>
-__device__ int64_t distinct_blocks(const int32_t *rp, const int32_t *ci, int32_t m, int32_t rb) {
-  int32_t pos[B], end[B];
-#pragma unroll
-  for (int i = 0; i < B; ++i) {
-    const int32_t r = rb * B + i;
-    pos[i] = r < m ? rp[r] : 0;
-    end[i] = r < m ? rp[r + 1] : 0;
-  }
-  int64_t count = 0;
-  while (true) {
-    int32_t best = INT32_MAX;
-#pragma unroll
-    for (int i = 0; i < B; ++i)
-      if (pos[i] < end[i]) best = min(best, ci[pos[i]] / B);
-    if (best == INT32_MAX) break;
-    ++count;
-#pragma unroll
-    for (int i = 0; i < B; ++i)
-      while (pos[i] < end[i] && ci[pos[i]] / B == best) ++pos[i];
-  }
-  return count;
-}
-
 __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci,
                                                       double *__restrict__ partial) {
@@ -61,8 +35,6 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *
       v[5] = fmax(fabs(double(mx) - double(r)), fabs(double(mn) - double(r)));
     }
   }
-  if (r < (int64_t(m) + 3) / 4) v[6] = double(distinct_blocks<4>(rp, ci, m, int32_t(r)));
-  if (r < (int64_t(m) + 7) / 8) v[7] = double(distinct_blocks<8>(rp, ci, m, int32_t(r)));
 #pragma unroll
   for (int k = 0; k < kStats; ++k) red[k][tid] = v[k];
   __syncthreads();
@@ -77,6 +49,71 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *
     __syncthreads();
   }
   if (tid < kStats) partial[int64_t(blockIdx.x) * kStats + tid] = red[tid][0];
+}
+
+__device__ __forceinline__ int32_t lower_bound_i32(const int32_t *a, int32_t lo, int32_t hi,
+                                                   int32_t key) {
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Distinct B-wide column blocks of every block row (the block-fill features),
+// one wave per block row: lane l merges only the entries whose block column
+// lies in its slice [l*S, (l+1)*S) of the nbc block columns, found by binary
+// search in each row, so a block row of long rows is split over the wave
+// (waves stride over the block rows).  Rows must have ascending columns
+// (unsorted rows over-count).  Writes one kStats record per workgroup with the
+// count in `slot`.
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_block_fill(int32_t m, int32_t nbc,
+                                                       const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       double *__restrict__ partial, int slot) {
+  __shared__ double red[kBlock / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t S = (nbc + 63) / 64;
+  const int32_t lo = lane * S, hi = min(nbc, lo + S);
+  int64_t count = 0;
+  const int64_t nbr = (int64_t(m) + B - 1) / B;
+  for (int64_t rb = int64_t(blockIdx.x) * (kBlock / 64) + wave; rb < nbr;
+       rb += int64_t(gridDim.x) * (kBlock / 64)) {
+    if (lo >= hi) continue;
+    int32_t pos[B], end[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const int64_t r = rb * B + i;
+      pos[i] = end[i] = 0;
+      if (r < m) {
+        const int32_t s0 = rp[r], s1 = rp[r + 1];
+        pos[i] = lower_bound_i32(ci, s0, s1, lo * B);
+        end[i] = lower_bound_i32(ci, pos[i], s1, int32_t(min<int64_t>(int64_t(hi) * B, INT32_MAX)));
+      }
+    }
+    while (true) {
+      int32_t best = INT32_MAX;
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        if (pos[i] < end[i]) best = min(best, ci[pos[i]] / B);
+      if (best == INT32_MAX) break;
+      ++count;
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        while (pos[i] < end[i] && ci[pos[i]] / B == best) ++pos[i];
+    }
+  }
+  const double c = group_sum_c<64>(double(count));
+  if (lane == 0) red[wave] = c;
+  __syncthreads();
+  if (threadIdx.x < kStats) {
+    double t = 0;
+    if (int(threadIdx.x) == slot)
+      for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+    partial[int64_t(blockIdx.x) * kStats + threadIdx.x] = t;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fold_stats(int64_t nparts, const double *partial,
@@ -107,11 +144,19 @@ __global__ __launch_bounds__(kBlock) void k_fold_stats(int64_t nparts, const dou
 hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s) {
   const unsigned grid = grid_for(A.m);
   if (grid == 0) return hipMemsetAsync(stats_dev, 0, sizeof(double) * kStats, s);
-  pls::DevBuf part;
-  hipError_t e = part.alloc(sizeof(double) * kStats * grid);
+  constexpr int kWaves = kBlock / 64;  // block rows per workgroup of the fill kernels
+  const unsigned g4 = unsigned(std::min<int64_t>(2048, (int64_t(A.m) + 4 * kWaves - 1) / (4 * kWaves)));
+  const unsigned g8 = unsigned(std::min<int64_t>(2048, (int64_t(A.m) + 8 * kWaves - 1) / (8 * kWaves)));
+  pls::DevBuf part;  // kStats records: row statistics, then 4x4 and 8x8 block counts
+  hipError_t e = part.alloc(sizeof(double) * kStats * (size_t(grid) + g4 + g8));
   if (e != hipSuccess) return e;
-  k_row_stats<<<grid, kBlock, 0, s>>>(A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), part.as<double>());
-  k_fold_stats<<<1, kBlock, 0, s>>>(grid, part.as<double>(), stats_dev);
+  double *p = part.as<double>();
+  k_row_stats<<<grid, kBlock, 0, s>>>(A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), p);
+  k_block_fill<4><<<g4, kBlock, 0, s>>>(A.m, int32_t((int64_t(A.n) + 3) / 4), A.rp.as<int32_t>(),
+                                        A.ci.as<int32_t>(), p + size_t(kStats) * grid, 6);
+  k_block_fill<8><<<g8, kBlock, 0, s>>>(A.m, int32_t((int64_t(A.n) + 7) / 8), A.rp.as<int32_t>(),
+                                        A.ci.as<int32_t>(), p + size_t(kStats) * (grid + g4), 7);
+  k_fold_stats<<<1, kBlock, 0, s>>>(int64_t(grid) + g4 + g8, p, stats_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(s);  // `part` is freed on return

@@ -164,3 +164,20 @@ def test_shipped_model_picks_like_numpy(pls, gen, tmp_path):
         cand[11] = f[10] >= 0.75 and mm % 8 == 0
         want = int(np.argmin(np.where(cand, pred[0], np.inf)))
         assert A.variant() == want, (kind, pls.VARIANTS[A.variant()], pls.VARIANTS[want])
+
+
+def test_block_fill_features_match_numpy(pls):
+    """4x4 / 8x8 block fill against a direct count of distinct (block row, block
+    column) pairs, with long rows that the fill kernel splits over a wave."""
+    rng = np.random.default_rng(31)
+    m, n = 4096, 3000
+    lens = rng.integers(0, 30, m)
+    lens[::37] = 2500
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(l), replace=False)) for l in lens]).astype(np.int32)
+    A = pls.SpMat.csr(m, n, rp, ci, np.ones(len(ci)))
+    f = A.features()
+    rows = np.repeat(np.arange(m), lens)
+    for b, idx in ((4, 9), (8, 10)):
+        blocks = len(np.unique((rows // b).astype(np.int64) * (n // b + 1) + ci // b))
+        assert abs(f[idx] - len(ci) / (blocks * b * b)) <= 1e-6 * f[idx]
