@@ -374,6 +374,7 @@ def main() -> int:
             "row_length": "round(lognormal(2.5, 1.0)) clipped to [1, 20000]",
             "alpha": 1.0, "beta": 0.0, "seed": SEED_A, "variant": chosen,
             "hbm_fraction": round(value / HBM_PEAK_GBS, 4),
+            "gflops": round(2.0 * nnz / (wall_ms * 1e-3) / 1e9, 1),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
