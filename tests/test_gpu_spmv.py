@@ -404,3 +404,36 @@ def test_coo_and_csc_objects(pls, orc):
     y = y0.copy()
     Bc.spmv(x, y, 0.5, 2.0)
     assert rel_err(y, ref) <= TOL64
+
+
+def test_csc_and_coo_goldens(pls, golden):
+    g = golden["csc_kernel_paths_d"]
+    A = pls.SpMat.csc(g["m"], g["n"], g["row_indx"], g["col_ptr"], np.array(g["vals"]))
+    y = np.zeros(4)
+    A.spmv(np.array(g["x"]), y)
+    assert np.max(np.abs(y - g["expected_notrans"])) <= g["tol"]
+    y = np.zeros(4)
+    A.spmv(np.array(g["x"]), y, trans=pls.TRANS)
+    assert np.max(np.abs(y - g["expected_trans"])) <= g["tol"]
+    u = g["update"]
+    A.update([u["row"]], [u["col"]], [u["value"]])
+    _, _, rp, ci, v = A.export_csr(0)
+    D = np.zeros((4, 4))
+    for i in range(4):
+        D[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    assert np.array_equal(D.reshape(-1), g["dense_after_update_row_major"])
+    c = golden["coo_spmv_paths_real_part"]
+    B = pls.SpMat.coo(c["m"], c["n"], c["row_indx"], c["col_indx"], np.array(c["vals"]))
+    assert B.query() == (0, 4, 4, 5)
+    y = np.zeros(4)
+    B.spmv(np.array(c["x"]), y)
+    assert np.max(np.abs(y - c["expected_notrans"])) <= c["tol"]
+    y = np.zeros(4)
+    B.spmv(np.array(c["x"]), y, trans=pls.TRANS)
+    assert np.max(np.abs(y - c["expected_trans"])) <= c["tol"]
+    u = c["update"]
+    B.update([u["row"]], [u["col"]], [u["value"]])
+    y = np.zeros(4)
+    B.spmv(np.array(c["x"]), y)
+    assert np.max(np.abs(y - c["expected_after_update"])) <= c["tol"]
+    assert B.update([c["bad_update"]["row"]], [c["bad_update"]["col"]], [1.0], raise_on_error=False) == 1
