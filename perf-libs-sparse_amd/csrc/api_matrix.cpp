@@ -489,9 +489,21 @@ perflibs_status_t update_error(perflibs_spmat_top_t *A, int64_t pos) {
           "existing values, it cannot be used to introduce new non-zeros.");
 }
 
+// Device plan after a value update: the updated entries are written into the
+// plan's device layout in place (update.cpp); a plan whose layout cannot be
+// patched is dropped and rebuilt from the host values by the next execution,
+// keeping the chosen variant.
+void refresh_plan(perflibs_spmat_top_t *A, const std::vector<pls::ValueHit> &hits) {
+  if (pls::patch_plan(A, hits)) return;
+  const int keep = A->plan.variant;
+  A->plan.clear();
+  if (keep >= 0 && A->forced_variant < 0 && A->forced_C < 0) A->forced_variant = keep;
+}
+
 // Value update (src/matvec.cpp:530-562; CSR search of
-// src/compressed_sparse_rows.cpp:3059-3098).  Device plans are rebuilt from
-// the updated host values on the next execution, keeping the chosen variant.
+// src/compressed_sparse_rows.cpp:3059-3098).  Updates are applied in order;
+// at the first one that names no stored entry the call fails, with the
+// updates before it applied (host values and device plan alike).
 template <class T>
 perflibs_status_t update(perflibs_spmat_t A, perflibs_int_t nu, const perflibs_int_t *row_indx,
                          const perflibs_int_t *col_indx, const T *vals) {
@@ -500,20 +512,25 @@ perflibs_status_t update(perflibs_spmat_t A, perflibs_int_t nu, const perflibs_i
   if (nu < 0) return pls::param_fail(A, 1, "The number of updates can not be negative.");
   if (A->dt != dtype_of<T>()) return pls::param_fail(A, 1, "datatype mismatch");
   const int32_t base = A->index_base;
+  std::vector<pls::ValueHit> hits;
   if (A->fmt == Fmt::Csr || A->fmt == Fmt::Scs) {
     if (!A->nocopy && A->csr.vals == nullptr) return PERFLIBS_STATUS_EXECUTION_FAILURE;
     T *vv = const_cast<T *>(static_cast<const T *>(A->csr.vals));
     for (int32_t u = 0; u < nu; ++u) {
       const int32_t r = row_indx[u] - base, c = col_indx[u] - base;
-      if (r < 0 || r > A->m - 1 || c < 0 || c > A->n - 1) return update_error(A, int64_t(u) + base);
       bool ok = false;
-      for (int32_t j = A->csr.rp[r] - base; j < A->csr.rp[r + 1] - base; ++j)
-        if (A->csr.ci[j] == col_indx[u]) {
-          vv[j] = vals[u];
-          ok = true;
-          break;
-        }
-      if (!ok) return update_error(A, int64_t(u) + base);
+      if (r >= 0 && r <= A->m - 1 && c >= 0 && c <= A->n - 1)
+        for (int32_t j = A->csr.rp[r] - base; j < A->csr.rp[r + 1] - base; ++j)
+          if (A->csr.ci[j] == col_indx[u]) {
+            vv[j] = vals[u];
+            hits.push_back({r, c, int64_t(j)});
+            ok = true;
+            break;
+          }
+      if (!ok) {
+        refresh_plan(A, hits);
+        return update_error(A, int64_t(u) + base);
+      }
     }
   } else if (A->fmt == Fmt::Bsr) {
     pls::HostBsr &B = A->bsr;
@@ -521,16 +538,23 @@ perflibs_status_t update(perflibs_spmat_t A, perflibs_int_t nu, const perflibs_i
     const int64_t bb = int64_t(B.b) * B.b;
     for (int32_t u = 0; u < nu; ++u) {
       const int32_t r = row_indx[u] - base, c = col_indx[u] - base;
-      if (r < 0 || r > A->m - 1 || c < 0 || c > A->n - 1) return update_error(A, int64_t(u) + base);
-      const int32_t rb = r / B.b, cb = c / B.b, i = r % B.b, j = c % B.b;
       bool ok = false;
-      for (int32_t k = B.rp[rb] - base; k < B.rp[rb + 1] - base; ++k)
-        if (B.ci[k] - base == cb) {
-          vv[k * bb + (B.layout == PERFLIBS_COL_MAJOR ? i + int64_t(B.b) * j : int64_t(i) * B.b + j)] = vals[u];
-          ok = true;
-          break;
-        }
-      if (!ok) return update_error(A, int64_t(u) + base);
+      if (r >= 0 && r <= A->m - 1 && c >= 0 && c <= A->n - 1) {
+        const int32_t rb = r / B.b, cb = c / B.b, i = r % B.b, j = c % B.b;
+        for (int32_t k = B.rp[rb] - base; k < B.rp[rb + 1] - base; ++k)
+          if (B.ci[k] - base == cb) {
+            const int64_t at = k * bb + (B.layout == PERFLIBS_COL_MAJOR ? i + int64_t(B.b) * j
+                                                                        : int64_t(i) * B.b + j);
+            vv[at] = vals[u];
+            hits.push_back({r, c, at});
+            ok = true;
+            break;
+          }
+      }
+      if (!ok) {
+        refresh_plan(A, hits);
+        return update_error(A, int64_t(u) + base);
+      }
     }
   } else if (A->fmt == Fmt::Dense) {
     pls::Dense &D = A->dense;
@@ -543,13 +567,12 @@ perflibs_status_t update(perflibs_spmat_t A, perflibs_int_t nu, const perflibs_i
     }
     D.dev_valid = false;
     D.matrix_is_zero = false;
+    A->plan.clear();
+    return PERFLIBS_STATUS_SUCCESS;
   } else {
     return PERFLIBS_STATUS_EXECUTION_FAILURE;
   }
-  // keep the variant, drop the stale device values
-  const int keep = A->plan.variant;
-  A->plan.clear();
-  if (keep >= 0 && A->forced_variant < 0 && A->forced_C < 0) A->forced_variant = keep;
+  refresh_plan(A, hits);
   return PERFLIBS_STATUS_SUCCESS;
 }
 

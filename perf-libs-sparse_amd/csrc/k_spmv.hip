@@ -430,17 +430,25 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, acc_t<X> alpha, con
     y[i] = X(r);
   }
 }
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even
+  return uint16_t(u >> 16);
+}
 __global__ __launch_bounds__(kBlock) void k_f32_to_bf16(const float *src, uint16_t *dst,
                                                         int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) dst[i] = bf16_rne(src[i]);
+}
+// In-place value update: vals[slot[i]] = v[i] in the storage type.
+template <class S, class V>
+__global__ __launch_bounds__(kBlock) void k_patch_vals(int64_t n, const int64_t *slot,
+                                                       const V *v, S *vals) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
-  uint32_t u = __float_as_uint(src[i]);
-  if ((u & 0x7fffffffu) > 0x7f800000u) {
-    dst[i] = uint16_t((u >> 16) | 0x40u);  // quiet NaN
-  } else {
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even
-    dst[i] = uint16_t(u >> 16);
-  }
+  if constexpr (sizeof(S) == 2) vals[slot[i]] = bf16_rne(v[i]);
+  else vals[slot[i]] = S(v[i]);
 }
 
 // ===========================================================================
@@ -662,6 +670,21 @@ hipError_t axpby_vec(bool dbl, int64_t n, double alpha, const void *x, double be
 hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_f32_to_bf16<<<grid_for(n), kBlock, 0, s>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t patch_vals(VStore vs, void *vals, const int64_t *slot, const void *v, int64_t n,
+                      hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (vs == VStore::F64)
+    k_patch_vals<double, double><<<grid_for(n), kBlock, 0, s>>>(
+        n, slot, static_cast<const double *>(v), static_cast<double *>(vals));
+  else if (vs == VStore::F32)
+    k_patch_vals<float, float><<<grid_for(n), kBlock, 0, s>>>(
+        n, slot, static_cast<const float *>(v), static_cast<float *>(vals));
+  else
+    k_patch_vals<uint16_t, float><<<grid_for(n), kBlock, 0, s>>>(
+        n, slot, static_cast<const float *>(v), static_cast<uint16_t *>(vals));
   return hipGetLastError();
 }
 

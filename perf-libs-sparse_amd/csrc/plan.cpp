@@ -222,8 +222,8 @@ perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int
 
 static perflibs_status_t upload_bsr(perflibs_spmat_top_t *A, int layout, int32_t b, int32_t mb,
                                     int32_t nb, const int32_t *rp, const int32_t *ci,
-                                    const void *vals, int op, int variant) {
-  auto d = std::make_unique<DevBsr>();
+                                    const void *vals, int op, int variant,
+                                    std::unique_ptr<DevBsr> d = std::make_unique<DevBsr>()) {
   d->layout = layout;
   d->b = b;
   d->mb = mb;
@@ -325,6 +325,8 @@ static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand
   if (e == hipSuccess) e = d->idx.upload(P.idx.data(), sizeof(uint32_t) * P.idx.size());
   if (e == hipSuccess) e = upload_vals(d->vals, P.vals.data(), int64_t(P.idx.size()), A->dt, vs);
   if (e != hipSuccess) return hip_fail(A, e, "column-panel plan upload");
+  d->h_tile_row0 = std::move(P.tile_row0);
+  d->h_seg = std::move(P.seg);
   A->plan.clear();
   A->plan.panel = std::move(d);
   A->plan.variant = variant;
@@ -353,8 +355,12 @@ perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op, 
                      bci, bvd, bvf))
       return exec_fail(A, "BSR variant needs dimensions divisible by the block size");
     const void *bv = A->dt == DType::F64 ? (const void *)bvd.data() : (const void *)bvf.data();
+    auto d = std::make_unique<DevBsr>();
+    d->from_csr = true;
+    d->h_rp = brp;
+    d->h_ci = bci;
     return upload_bsr(A, PERFLIBS_ROW_MAJOR, b, o.m / b, o.n / b, brp.data(), bci.data(), bv, op,
-                      variant);
+                      variant, std::move(d));
   }
   return exec_fail(A, "unknown SpMV variant");
 }

@@ -136,6 +136,9 @@ struct DevPanel {
   DevBuf seg;        // int64 [ntiles*npanels+1], quad-aligned start | padding count
   DevBuf idx;        // uint32 [padded nnz], (row - tile row0) << pw_log2 | col - panel col0
   DevBuf vals;       // [padded nnz]
+  // host copies kept to locate entries for in-place value updates
+  std::vector<int32_t> h_tile_row0;
+  std::vector<int64_t> h_seg;
 };
 
 struct DevBsr {
@@ -144,6 +147,10 @@ struct DevBsr {
   int64_t nnzb = 0;
   VStore vs = VStore::F64;
   DevBuf rp, ci, vals;
+  // true when the blocks were built from a CSR operand (row-major blocks that
+  // sum duplicate entries); h_rp / h_ci then keep the 0-based block structure
+  bool from_csr = false;
+  std::vector<int32_t> h_rp, h_ci;
 };
 
 // Which operand the device plan was built for.
@@ -288,6 +295,20 @@ perflibs_status_t launch_plan(perflibs_spmat_top_t *A, double alpha,
                               const void *x, double beta, void *y);
 VStore value_store(const perflibs_spmat_top_t *A);
 
+// ---- in-place value updates (update.cpp) ----
+// One updated entry of the handle: 0-based row and column of A and the index
+// of its value in the handle's host value array (CSR position, or the value
+// index inside the BSR value array).
+struct ValueHit {
+  int32_t r, c;
+  int64_t j;
+};
+// Write the updated host values of `hits` into the device plan in place.
+// Returns false when the plan's layout cannot be patched (a transposed
+// operand, or a CSR-family plan of a BSR handle, whose expansion drops zero
+// values); the caller then drops the plan so the next execution rebuilds it.
+bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits);
+
 // ---- selection (select.cpp) ----
 int pick_variant(perflibs_spmat_top_t *A, const Operand &o);
 perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o,
@@ -333,6 +354,11 @@ hipError_t strided_copy(bool dbl, int64_t n, const void *src, int64_t ss, void *
                         hipStream_t s);
 // FP32 -> BF16 (round to nearest even), n values, device to device.
 hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s);
+// vals[slot[i]] = v[i] for i < n, converted to the storage type vs (v holds
+// doubles for F64 storage, floats for F32 and BF16).  Slots must be distinct
+// or carry equal values.
+hipError_t patch_vals(VStore vs, void *vals, const int64_t *slot, const void *v, int64_t n,
+                      hipStream_t s);
 // Row statistics of a 0-based device CSR for the selector features:
 // out[0..] partial sums reduced on the device, see select.cpp.
 hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s);
