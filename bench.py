@@ -279,6 +279,61 @@ def spmm_bench(args) -> int:
     return 0
 
 
+def mtx_bench(args) -> int:
+    """FP64 SpMV on a Matrix Market file (read like the reference's benchmark
+    driver, bench/pl_bench_sparse_x_dense.c:124-240): optimize() selects a
+    variant, every applicable variant is timed for the regret, x uniform[-1, 1]."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from plsparse import workloads as W
+
+    t0 = time.perf_counter()
+    m, n, rp, ci, v = W.read_mtx(args.mtx)
+    nnz = int(rp[-1])
+    log(f"{args.mtx}: {m} x {n}, nnz={nnz}, read in {time.perf_counter() - t0:.1f}s")
+    nbytes = nnz * (8 + 4) + (m + 1) * 4 + n * 8 + m * 8
+    P.set_stream(torch.cuda.current_stream().cuda_stream)
+    xd = torch.from_numpy(W.uniform(n, SEED_X)).cuda()
+    yd = torch.zeros(m, dtype=torch.float64, device="cuda")
+    A = P.SpMat.csr(m, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+    A.hint(P.HINT_SPMV_INVOCATIONS, P.INVOCATIONS_MANY)
+    A.optimize()
+    chosen = P.VARIANTS[A.variant()]
+    wall_ms, kern_ms = time_gpu(A, xd, yd, args.warmup, args.steps)
+    A.destroy()
+    sweep = {}
+    for vi, name in enumerate(P.VARIANTS):
+        b = 4 if name == "bsr_b4" else 8 if name == "bsr_b8" else 1
+        if m % b or n % b:
+            continue
+        B = P.SpMat.csr(m, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+        B.force_variant(vi)
+        B.optimize()
+        sweep[name] = time_gpu(B, xd, yd, 3, 20)[1]
+        B.destroy()
+    best = min(sweep, key=sweep.get)
+    value = nbytes / (wall_ms * 1e-3) / 1e9
+    achieved = nbytes / (kern_ms * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": f"file {Path(args.mtx).name}",
+        "config": {"workload": f"spmv_fp64_mtx_{Path(args.mtx).stem}", "rows": m, "cols": n,
+                   "nnz": nnz, "alpha": 1.0, "beta": 0.0, "variant": chosen},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": chosen, "kernel_ms": round(kern_ms, 4), "algorithmic_bytes": nbytes},
+        "cpu_baseline": None,
+        "variants_gbs": {k: round(nbytes / ms / 1e6, 1) for k, ms in sweep.items()},
+        "selector": {"picked": chosen, "exhaustive_best": best,
+                     "regret": round(sweep[chosen] / sweep[best] - 1.0, 4)},
+    }), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +347,7 @@ def main() -> int:
                     help="spmv: the headline SpMV line (default); batch: the batched configuration; "
                          "spmm: CSR/BSR x dense on the 27-point stencil")
     ap.add_argument("--batch", type=int, default=4096, help="problems of the batched workload")
+    ap.add_argument("--mtx", default=None, help="FP64 SpMV on this Matrix Market file instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU is not part of this build",
@@ -301,6 +357,8 @@ def main() -> int:
         return batch_bench(args)
     if args.workload == "spmm":
         return spmm_bench(args)
+    if args.mtx:
+        return mtx_bench(args)
 
     import numpy as np
     import torch

@@ -15,7 +15,8 @@
  *   execute : perflibs_spmv_exec_{s,d}, perflibs_spmm_exec_{s,d}
  *                                                 (lines 464-469, 483-492)
  *   update  : perflibs_spmat_update_{s,d}                   (lines 561-570)
- *   export  : perflibs_spmat_export_{csr,dense}_{s,d}  (lines 582-592, 640-645)
+ *   export  : perflibs_spmat_export_{csr,dense,bsr}_{s,d}
+ *                                        (lines 582-592, 640-661)
  *   errors  : perflibs_spmat_print_err                       (line 769)
  *
  * Semantics are the reference's: status 0/1/2, parameter-position error codes,
@@ -225,6 +226,19 @@ perflibs_status_t perflibs_spmat_export_dense_s(
 perflibs_status_t perflibs_spmat_export_dense_d(
     perflibs_const_spmat_t A, enum perflibs_dense_layout layout,
     perflibs_int_t *m, perflibs_int_t *n, double **vals);
+/* BSR export (reference include/perflibs_sparse.h:652-661): blocks of a BSR
+ * object in the requested base and block layout; any other object as 1 x 1
+ * blocks of its CSR form. */
+perflibs_status_t perflibs_spmat_export_bsr_s(
+    perflibs_const_spmat_t A, enum perflibs_dense_layout block_layout,
+    perflibs_int_t index_base, perflibs_int_t *m, perflibs_int_t *n,
+    perflibs_int_t *block_size, perflibs_int_t **row_ptr,
+    perflibs_int_t **col_indx, float **vals);
+perflibs_status_t perflibs_spmat_export_bsr_d(
+    perflibs_const_spmat_t A, enum perflibs_dense_layout block_layout,
+    perflibs_int_t index_base, perflibs_int_t *m, perflibs_int_t *n,
+    perflibs_int_t *block_size, perflibs_int_t **row_ptr,
+    perflibs_int_t **col_indx, double **vals);
 
 /* ---- errors (reference include/perflibs_sparse.h:769) ---- */
 void perflibs_spmat_print_err(perflibs_spmat_t A);

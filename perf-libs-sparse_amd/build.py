@@ -62,7 +62,21 @@ def build_workloads() -> Path:
     out = LIB / "libplsparse_workloads.so"
     tmp = LIB / "libplsparse_workloads.so.tmp"
     _run(["g++", "-O3", "-fPIC", "-shared", "-fopenmp", "-std=c++17",
-          str(PKG / "workloads" / "gen.cpp"), "-o", str(tmp)])
+          str(PKG / "workloads" / "gen.cpp"), str(PKG / "workloads" / "mtx.cpp"),
+          "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_apps() -> Path:
+    """The C benchmark driver (apps/sparse_x_dense.c), linked against both
+    in-tree libraries with an $ORIGIN rpath so it runs from lib/."""
+    LIB.mkdir(exist_ok=True)
+    out = LIB / "sparse_x_dense"
+    tmp = LIB / "sparse_x_dense.tmp"
+    _run(["gcc", "-O2", "-std=c99", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'workloads'}",
+          str(PKG / "apps" / "sparse_x_dense.c"), f"-L{LIB}", "-lplsparse_amd",
+          "-lplsparse_workloads", "-lm", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 
@@ -70,6 +84,7 @@ def build_workloads() -> Path:
 def main() -> None:
     print(build_library(), flush=True)
     print(build_workloads(), flush=True)
+    print(build_apps(), flush=True)
 
 
 if __name__ == "__main__":

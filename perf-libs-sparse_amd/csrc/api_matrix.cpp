@@ -407,6 +407,59 @@ perflibs_status_t export_csr(perflibs_const_spmat_t Ac, perflibs_int_t index_bas
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+// BSR export (src/export.cpp:566-735): a BSR object gives its blocks in the
+// requested base and block layout (a block stored in the other layout is
+// transposed); any other object is first converted to CSR as for
+// export_csr and then exported as BSR with 1 x 1 blocks, which is the
+// reference's csr2bsr (src/block_sparse_rows.cpp:350-370).
+template <class T>
+perflibs_status_t export_bsr(perflibs_const_spmat_t Ac, enum perflibs_dense_layout block_layout,
+                             perflibs_int_t index_base, perflibs_int_t *m, perflibs_int_t *n,
+                             perflibs_int_t *block_size, perflibs_int_t **row_ptr,
+                             perflibs_int_t **col_indx, T **vals) {
+  auto *A = const_cast<perflibs_spmat_top_t *>(Ac);
+  if (!A || !m || !n || !block_size || !row_ptr || !col_indx || !vals)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (A->dt != dtype_of<T>() && A->fmt != Fmt::Null && A->fmt != Fmt::Identity)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (A->m <= 0 || A->n <= 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (index_base != 0 && index_base != 1) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (block_layout != PERFLIBS_COL_MAJOR && block_layout != PERFLIBS_ROW_MAJOR)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (A->fmt != Fmt::Bsr) {
+    const perflibs_status_t st = export_csr<T>(Ac, index_base, m, n, row_ptr, col_indx, vals);
+    if (st == PERFLIBS_STATUS_SUCCESS) *block_size = 1;
+    return st;
+  }
+  const pls::HostBsr &B = A->bsr;
+  const int32_t b = B.b, mb = B.mb, base = B.rp[0];
+  const int64_t nnzb = int64_t(B.rp[mb]) - base, bb = int64_t(b) * b;
+  auto *orp = static_cast<perflibs_int_t *>(std::malloc(sizeof(perflibs_int_t) * (size_t(mb) + 1)));
+  auto *oci = static_cast<perflibs_int_t *>(std::malloc(sizeof(perflibs_int_t) * size_t(nnzb ? nnzb : 1)));
+  auto *ov = static_cast<T *>(std::malloc(sizeof(T) * size_t(nnzb ? nnzb * bb : 1)));
+  if (!orp || !oci || !ov) {
+    std::free(orp);
+    std::free(oci);
+    std::free(ov);
+    return PERFLIBS_STATUS_EXECUTION_FAILURE;
+  }
+  for (int32_t i = 0; i <= mb; ++i) orp[i] = B.rp[i] - base + index_base;
+  for (int64_t k = 0; k < nnzb; ++k) oci[k] = B.ci[k] - base + index_base;
+  const T *sv = static_cast<const T *>(B.vals);
+  const bool same = block_layout == B.layout;
+  for (int64_t k = 0; k < nnzb; ++k)
+    for (int32_t i = 0; i < b; ++i)
+      for (int32_t j = 0; j < b; ++j)
+        ov[k * bb + int64_t(i) * b + j] = sv[k * bb + (same ? int64_t(i) * b + j : i + int64_t(b) * j)];
+  *m = A->m;
+  *n = A->n;
+  *block_size = b;
+  *row_ptr = orp;
+  *col_indx = oci;
+  *vals = ov;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 // Dense export (src/export.cpp:512-549): malloc'd m x n copy, packed.
 template <class T>
 perflibs_status_t export_dense(perflibs_const_spmat_t Ac, enum perflibs_dense_layout layout,
@@ -796,6 +849,22 @@ perflibs_status_t perflibs_spmat_export_csr_d(perflibs_const_spmat_t A, perflibs
                                               perflibs_int_t **row_ptr, perflibs_int_t **col_indx,
                                               double **vals) {
   return export_csr<double>(A, index_base, m, n, row_ptr, col_indx, vals);
+}
+perflibs_status_t perflibs_spmat_export_bsr_s(perflibs_const_spmat_t A,
+                                              enum perflibs_dense_layout block_layout,
+                                              perflibs_int_t index_base, perflibs_int_t *m,
+                                              perflibs_int_t *n, perflibs_int_t *block_size,
+                                              perflibs_int_t **row_ptr, perflibs_int_t **col_indx,
+                                              float **vals) {
+  return export_bsr<float>(A, block_layout, index_base, m, n, block_size, row_ptr, col_indx, vals);
+}
+perflibs_status_t perflibs_spmat_export_bsr_d(perflibs_const_spmat_t A,
+                                              enum perflibs_dense_layout block_layout,
+                                              perflibs_int_t index_base, perflibs_int_t *m,
+                                              perflibs_int_t *n, perflibs_int_t *block_size,
+                                              perflibs_int_t **row_ptr, perflibs_int_t **col_indx,
+                                              double **vals) {
+  return export_bsr<double>(A, block_layout, index_base, m, n, block_size, row_ptr, col_indx, vals);
 }
 perflibs_status_t perflibs_spmat_export_dense_s(perflibs_const_spmat_t A,
                                                 enum perflibs_dense_layout layout,

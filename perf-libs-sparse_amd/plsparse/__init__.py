@@ -84,6 +84,8 @@ _SIGS = {
     "perflibs_spmat_update_d": (_I, [_P, _I, _P, _P, _P]),
     "perflibs_spmat_export_csr_s": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
     "perflibs_spmat_export_csr_d": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_bsr_s": (_I, [_P, _I, _I, _P, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_bsr_d": (_I, [_P, _I, _I, _P, _P, _P, _PP, _PP, _PP]),
     "perflibs_spmat_export_dense_s": (_I, [_P, C.c_int, _P, _P, _PP]),
     "perflibs_spmat_export_dense_d": (_I, [_P, C.c_int, _P, _P, _PP]),
     "perflibs_spmat_print_err": (None, [_P]),
@@ -349,6 +351,26 @@ class SpMat:
         for p in (rp, ci, v):
             libc.free(p)
         return mm, n.value, rpa, cia, va
+
+    def export_bsr(self, block_layout=ROW_MAJOR, index_base=0):
+        """(m, n, block_size, row_ptr, col_indx, vals[nnzb * b * b])."""
+        m, n, b = C.c_int32(), C.c_int32(), C.c_int32()
+        rp, ci, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        fn = f"perflibs_spmat_export_bsr_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(self.h, block_layout, index_base, C.byref(m), C.byref(n),
+                                     C.byref(b), C.byref(rp), C.byref(ci), C.byref(v)))
+        libc = C.CDLL(None)
+        bs = b.value
+        mb = m.value // bs
+        rpa = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int32)), (mb + 1,)).copy()
+        nnzb = int(rpa[-1] - rpa[0])
+        cia = np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_int32)), (max(nnzb, 1),))[:nnzb].copy()
+        ctype = C.c_double if self.dtype == np.float64 else C.c_float
+        nv = nnzb * bs * bs
+        va = np.ctypeslib.as_array(C.cast(v, C.POINTER(ctype)), (max(nv, 1),))[:nv].copy()
+        for p in (rp, ci, v):
+            libc.free(p)
+        return m.value, n.value, bs, rpa, cia, va
 
     def export_dense(self, layout=ROW_MAJOR):
         m, n = C.c_int32(), C.c_int32()

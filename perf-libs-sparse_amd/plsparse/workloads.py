@@ -44,6 +44,16 @@ _SIGS = {
 }
 
 
+class _MtxCsr(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int64),
+                ("row_ptr", C.POINTER(C.c_int32)), ("col_indx", C.POINTER(C.c_int32)),
+                ("vals", C.POINTER(C.c_double))]
+
+
+_SIGS["pls_mtx_read_csr"] = (C.c_int, [C.c_char_p, C.POINTER(_MtxCsr), C.c_char_p, C.c_size_t])
+_SIGS["pls_mtx_free"] = (None, [C.POINTER(_MtxCsr)])
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -202,3 +212,22 @@ def banded_batch(nprob: int, seed: int, rows=(256, 65536), bw=(32, 512), nnz=(5,
         rp, ci, v = banded(m, w, nnz[0], nnz[1], (seed * 1_000_003 + p) & 0xFFFFFFFFFFFF)
         out.append((m, rp, ci, v, k))
     return out
+
+
+def read_mtx(path) -> tuple[int, int, np.ndarray, np.ndarray, np.ndarray]:
+    """Matrix Market coordinate file -> (m, n, row_ptr, col_indx, vals), 0-based
+    CSR sorted by (row, column), through workloads/mtx.cpp (the acceptance
+    rules of the reference's benchmark reader).  Raises ValueError with the
+    reader's message on a malformed file."""
+    a = _MtxCsr()
+    err = C.create_string_buffer(256)
+    if lib().pls_mtx_read_csr(str(path).encode(), C.byref(a), err, len(err)) != 0:
+        raise ValueError(err.value.decode())
+    try:
+        m, nnz = a.m, a.nnz
+        rp = np.ctypeslib.as_array(a.row_ptr, (m + 1,)).copy()
+        ci = np.ctypeslib.as_array(a.col_indx, (max(nnz, 1),))[:nnz].copy()
+        v = np.ctypeslib.as_array(a.vals, (max(nnz, 1),))[:nnz].copy()
+        return m, a.n, rp, ci, v
+    finally:
+        lib().pls_mtx_free(C.byref(a))

@@ -107,6 +107,45 @@ def test_export_bsr_as_csr_matches_oracle(pls, orc):
         assert np.array_equal(rp, erp) and np.array_equal(ci, eci) and np.array_equal(v, ev)
 
 
+def test_export_bsr_goldens(pls, golden):
+    for g in golden["export_bsr"]:
+        A = pls.SpMat.bsr(g["layout"], g["m"], g["n"], g["block_size"], g["row_ptr"],
+                          g["col_indx"], np.array(g["vals"]))
+        m, n, b, rp, ci, v = A.export_bsr(g["export_layout"], g["export_base"])
+        assert (m, n, b) == (g["m"], g["n"], g["block_size"])
+        assert rp.tolist() == g["expected_row_ptr"] and ci.tolist() == g["expected_col_indx"]
+        assert v.tolist() == g["expected_vals"]
+        if "expected_dense_row_major" in g:
+            assert A.export_dense(pls.ROW_MAJOR).reshape(-1).tolist() == g["expected_dense_row_major"]
+
+
+def test_export_bsr_layouts_and_other_formats(pls, orc):
+    rng = np.random.default_rng(3)
+    b, brp, bci = 3, [0, 2, 2, 3], [0, 1, 2]
+    vals = rng.standard_normal(3 * 9)
+    for layout in (pls.COL_MAJOR, pls.ROW_MAJOR):
+        A = pls.SpMat.bsr(layout, 9, 9, b, brp, bci, vals)
+        other = pls.ROW_MAJOR if layout == pls.COL_MAJOR else pls.COL_MAJOR
+        _, _, bs, rp, ci, v = A.export_bsr(other, 0)
+        assert bs == b and rp.tolist() == brp and ci.tolist() == bci
+        # every block transposed into the other layout, same dense image
+        assert np.array_equal(v.reshape(-1, b, b), vals.reshape(-1, b, b).transpose(0, 2, 1))
+        B = pls.SpMat.bsr(other, 9, 9, b, rp, ci, v)
+        assert np.array_equal(B.export_dense(), A.export_dense())
+    # a CSR object exports as 1 x 1 blocks of its CSR arrays (explicit zeros kept)
+    rp0, ci0, v0 = [1, 3, 3, 4], [1, 3, 2], [2.0, 0.0, -1.0]
+    C = pls.SpMat.csr(3, 3, rp0, ci0, v0)
+    m, n, bs, rp, ci, v = C.export_bsr(pls.COL_MAJOR, 0)
+    assert (m, n, bs) == (3, 3, 1)
+    assert rp.tolist() == [0, 2, 2, 3] and ci.tolist() == [0, 2, 1] and v.tolist() == v0
+    # a dense object converts to CSR first (zeros dropped), then 1 x 1 blocks
+    D = pls.SpMat.dense(pls.ROW_MAJOR, 2, 2, 2, np.array([0.0, 5.0, 6.0, 0.0]))
+    m, n, bs, rp, ci, v = D.export_bsr(pls.ROW_MAJOR, 1)
+    assert bs == 1 and rp.tolist() == [1, 2, 3] and ci.tolist() == [2, 1] and v.tolist() == [5.0, 6.0]
+    lib = pls.lib()
+    assert lib.perflibs_spmat_export_bsr_d(C.h, 77, 0, None, None, None, None, None, None) == 1
+
+
 def test_export_dense_of_sparse_and_dense(pls):
     A = pls.SpMat.csr(2, 3, [0, 2, 3], [0, 2, 1], [1.0, 2.0, 3.0])
     D = np.array([[1.0, 0.0, 2.0], [0.0, 3.0, 0.0]])
