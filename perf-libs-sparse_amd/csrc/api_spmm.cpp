@@ -132,11 +132,22 @@ template <class T> perflibs_status_t relayout(perflibs_spmat_top_t *D, int layou
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op) {
-  if (A->plan.csr && A->plan.op == op) return PERFLIBS_STATUS_SUCCESS;
+// CSR plan of op(A) for SpMM with k columns of B; for k >= 32 also the
+// column-compressed row blocks of the wide-B kernel (built once per plan).
+perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op, int64_t k) {
   pls::Operand o;
-  if (!pls::make_operand(A, op, o)) return pls::exec_fail(A, "matrix format has no GPU SpMM path");
-  return pls::build_plan(A, o, op, PERFLIBS_AMD_CSR_VECTOR);
+  if (!(A->plan.csr && A->plan.op == op)) {
+    if (!pls::make_operand(A, op, o)) return pls::exec_fail(A, "matrix format has no GPU SpMM path");
+    const perflibs_status_t st = pls::build_plan(A, o, op, PERFLIBS_AMD_CSR_VECTOR);
+    if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  }
+  if (k >= 32 && A->plan.csr->lb_state == 0) {
+    if (!o.prp && !pls::make_operand(A, op, o))
+      return pls::exec_fail(A, "matrix format has no GPU SpMM path");
+    const hipError_t e = pls::build_spmm_blocks(o, *A->plan.csr);
+    if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM block upload");
+  }
+  return PERFLIBS_STATUS_SUCCESS;
 }
 
 template <class T>
@@ -268,7 +279,7 @@ perflibs_status_t spmm(int transA, int transB, T alpha, perflibs_spmat_top_t *A,
   if (st == PERFLIBS_STATUS_SUCCESS) st = dense_to_dev(C);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   if (A->fmt != Fmt::Csr) pls::convert_to_csr(A);
-  st = csr_plan_for(A, transA);
+  st = csr_plan_for(A, transA, n);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   if (!exec) return PERFLIBS_STATUS_SUCCESS;
 
@@ -293,7 +304,7 @@ perflibs_status_t spmm_async(T alpha, perflibs_spmat_top_t *A, perflibs_int_t k,
   hipError_t e = pls::ensure_device();
   if (e != hipSuccess) return pls::hip_fail(A, e, "no usable HIP device");
   if (A->fmt != Fmt::Csr) pls::convert_to_csr(A);
-  perflibs_status_t st = csr_plan_for(A, PERFLIBS_SPARSE_OPERATION_NOTRANS);
+  perflibs_status_t st = csr_plan_for(A, PERFLIBS_SPARSE_OPERATION_NOTRANS, k);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   e = plk::csr_spmm(*A->plan.csr, dt == DType::F64, k, double(alpha), B, ldb, 1, double(beta), C,
                     ldc, 1, pls::stream());

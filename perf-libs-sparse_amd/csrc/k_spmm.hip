@@ -10,6 +10,9 @@
 // B(col_p, :) as one contiguous access and accumulates alpha*a_p*B in
 // registers; C is written once.  Strided (column-major / transposed) operands
 // use the generic kernel, one thread per output element.
+#include <algorithm>
+#include <type_traits>
+
 #include "k_common.hpp"
 #include "plsparse.hpp"
 
@@ -169,6 +172,114 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
   }
 }
 
+// Column-compressed row blocks (dispatched for k >= 32 when the plan's block
+// lists are short enough, see build_spmm_blocks): one workgroup per block of
+// consecutive rows.  The block's alpha-scaled values and one-byte list
+// positions are staged in LDS once; then, KC columns at a time, every B row the
+// block needs is loaded into LDS once (coalesced, KC contiguous elements per
+// row) and each row of C is accumulated from LDS by a group of KC lanes (one
+// column each), 8 non-zeros in flight per lane.  On a stencil a B row feeds
+// about 2.7 non-zeros of the block, so L2 -> CU traffic for B falls by that
+// factor against the direct kernels, which fetch a B row per non-zero.  A
+// block without a list (one row with more than UMAX distinct columns or
+// kLbNnz non-zeros) spreads its non-zeros over the workgroup's lane groups,
+// reads B directly and sums the groups through LDS.
+template <int KC, int UMAX, class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_spmm_lds(
+    const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
+    const int32_t *__restrict__ ucol, const uint8_t *__restrict__ lidx, int32_t k,
+    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const V *__restrict__ val,
+    const X *__restrict__ B, int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha,
+    acc_t<X> beta) {
+  using A = acc_t<X>;
+  constexpr int NG = kBlock / KC;  // lane groups (rows in flight) per workgroup
+  __shared__ A sB[UMAX * KC];
+  __shared__ A s_a[pls::kLbNnz];
+  __shared__ uint8_t s_l[pls::kLbNnz];
+  __shared__ int32_t s_rp[pls::kLbRows + 1];
+  __shared__ int32_t s_uc[UMAX];
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x, lane = tid % KC, grp = tid / KC;
+  const int32_t r0 = row0s[blk], r1 = row0s[blk + 1], nrows = r1 - r0;
+  const int32_t u0 = uoff[blk], nu = uoff[blk + 1] - u0;
+  const int32_t e0 = rp[r0], nnz = rp[r1] - e0;
+
+  if (nu == 0 && nnz > 0) {  // one long row without a list: B read directly
+    for (int32_t c0 = 0; c0 < k; c0 += KC) {
+      const int32_t col = c0 + lane;
+      A acc = 0;
+      if (col < k)
+        for (int32_t p = grp; p < nnz; p += NG)
+          acc += alpha * A(ldv(val, e0 + p)) * A(B[int64_t(ldi(ci, e0 + p)) * ldb + col]);
+      sB[grp * KC + lane] = acc;
+      __syncthreads();
+      if (grp == 0 && col < k) {
+        A r = 0;
+        for (int g = 0; g < NG; ++g) r += sB[g * KC + lane];
+        X *c = Cm + int64_t(r0) * ldc + col;
+        if (BETA) r += beta * A(*c);
+        *c = X(r);
+      }
+      __syncthreads();
+    }
+    return;
+  }
+
+  for (int32_t q = tid; q < nnz; q += kBlock) {
+    s_a[q] = alpha * A(ldv(val, e0 + q));
+    s_l[q] = lidx[e0 + q];
+  }
+  for (int32_t q = tid; q <= nrows; q += kBlock) s_rp[q] = rp[r0 + q] - e0;
+  for (int32_t q = tid; q < nu; q += kBlock) s_uc[q] = ucol[u0 + q];
+  __syncthreads();
+  int S = 1;  // groups per row: all groups busy, the S groups of a row in one wave
+  while (S * 2 * KC <= 64 && nrows * S * 2 <= NG) S *= 2;
+  const int sub = (tid % (KC * S)) / KC;
+  for (int32_t c0 = 0; c0 < k; c0 += KC) {
+    const int32_t col = c0 + lane;
+    // B rows of the list, columns [c0, c0 + KC): NG rows per pass, 4 passes
+    // of loads in flight before their LDS stores
+    for (int32_t i0 = grp; i0 < nu; i0 += 4 * NG) {
+      A t[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int32_t i = min(i0 + s * NG, nu - 1);
+        t[s] = col < k ? A(B[int64_t(s_uc[i]) * ldb + col]) : A(0);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (i0 + s * NG < nu) sB[(i0 + s * NG) * KC + lane] = t[s];
+    }
+    __syncthreads();
+    // S lane groups of one wave share a row when the block has few rows
+    // (long rows), each taking every S-th non-zero; xor shuffles fold them
+    for (int32_t rr = tid / (KC * S); rr < nrows; rr += kBlock / (KC * S)) {
+      const int32_t p0 = s_rp[rr], p1 = s_rp[rr + 1];
+      A acc = 0;
+      for (int32_t p = p0 + sub; p < p1; p += 8 * S) {
+        A a[8];
+        int32_t l[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int32_t pp = min(p + u * S, p1 - 1);
+          a[u] = p + u * S < p1 ? s_a[pp] : A(0);
+          l[u] = s_l[pp];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += a[u] * sB[l[u] * KC + lane];
+      }
+      for (int o = KC; o < KC * S; o <<= 1) acc += __shfl_xor(acc, o);
+      if (sub == 0 && col < k) {
+        X *c = Cm + int64_t(r0 + rr) * ldc + col;
+        A r = acc;
+        if (BETA) r += beta * A(*c);
+        *c = X(r);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_strided(
     int32_t m, int32_t k, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -233,6 +344,36 @@ void launch_rm(const pls::DevCsr &A, int32_t k, double alpha, const void *B, int
 }
 
 template <class V, class X>
+void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
+                double beta, void *C, int64_t ldc, hipStream_t s) {
+  auto go = [&](auto kc, auto um) {
+    constexpr int KC = decltype(kc)::value, UMAX = decltype(um)::value;
+    const auto *row0 = A.lb_row0.as<int32_t>();
+    const auto *uoff = A.lb_uoff.as<int32_t>();
+    const auto *ucol = A.lb_ucol.as<int32_t>();
+    const auto *lidx = A.lb_lidx.as<uint8_t>();
+    if (beta != 0.0)
+      k_spmm_lds<KC, UMAX, V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
+          row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
+    else
+      k_spmm_lds<KC, UMAX, V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
+          row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
+          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
+  };
+  // FP64: lists of up to 256 columns in 128-byte row chunks (16 columns);
+  // FP32: lists of up to 128 columns in chunks of 32 (k <= 32) or 64 columns.
+  // At most 32 KB of staged B; the best measured shapes on the 27-point
+  // stencil for each type.
+  if constexpr (sizeof(X) == 8)
+    go(std::integral_constant<int, 16>{}, std::integral_constant<int, pls::kLbUcols64>{});
+  else if (k <= 32)
+    go(std::integral_constant<int, 32>{}, std::integral_constant<int, pls::kLbUcols32>{});
+  else
+    go(std::integral_constant<int, 64>{}, std::integral_constant<int, pls::kLbUcols32>{});
+}
+
+template <class V, class X>
 void dispatch_rm(const pls::DevCsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
                  double beta, void *C, int64_t ldc, hipStream_t s) {
   if (k <= 1) launch_rm<1, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
@@ -258,8 +399,13 @@ hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha, con
   auto run = [&](auto vtag, auto xtag) -> hipError_t {
     using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
     using X = std::remove_pointer_t<decltype(xtag)>;
-    if (b_sc == 1 && c_sc == 1 && k <= 512) {
-      dispatch_rm<V, X>(A, k, alpha, B, b_sr, beta, C, c_sr, s);
+    if (b_sc == 1 && c_sc == 1 && k >= 32 && A.lb_state == 1 && A.lb_nblocks > 0) {
+      launch_lds<V, X>(A, k, alpha, B, b_sr, beta, C, c_sr, s);
+    } else if (b_sc == 1 && c_sc == 1) {
+      // row-major: column slices of at most 512 (the widest lane-group shape)
+      for (int32_t c0 = 0; c0 < k; c0 += 512)
+        dispatch_rm<V, X>(A, std::min<int32_t>(512, k - c0), alpha, static_cast<const X *>(B) + c0,
+                          b_sr, beta, static_cast<X *>(C) + c0, c_sr, s);
     } else {
       const unsigned grid = grid_for(int64_t(A.m) * k);
       if (beta != 0.0)

@@ -107,7 +107,27 @@ struct DevCsr {
   DevBuf blocks;  // int32 row starts of nnz-balanced row blocks (+ final m)
   int32_t nblocks = 0;
   int32_t vec_lanes = 8;  // lanes per row of the CSR-vector kernel
+  // Column-compressed row blocks of the wide-B SpMM kernel (built on the first
+  // SpMM with k >= 32): each block of consecutive rows lists its distinct
+  // columns once (ucol), and each non-zero stores its column's position in
+  // that list (lidx, one byte), so a workgroup loads every B row its block
+  // needs into LDS once per column chunk.  A block whose row alone exceeds the
+  // limits has no list (ucount 0) and reads B directly.
+  int lb_state = 0;  // 0: not built, 1: built and used, -1: built, too little reuse
+  int32_t lb_nblocks = 0;
+  int64_t lb_ucols = 0;  // total list length (sum of distinct columns per block)
+  DevBuf lb_row0;        // int32 [nb+1] first row of each block
+  DevBuf lb_uoff;        // int32 [nb+1] offsets into lb_ucol
+  DevBuf lb_ucol;        // int32 [lb_ucols]
+  DevBuf lb_lidx;        // uint8 [nnz]
 };
+// Limits of a column-compressed SpMM block (must match k_spmm.hip): distinct
+// columns for FP64 and for FP32/BF16 plans (list positions fit a byte),
+// non-zeros and rows.
+constexpr int32_t kLbUcols64 = 256;
+constexpr int32_t kLbUcols32 = 128;
+constexpr int32_t kLbNnz = 1024;
+constexpr int32_t kLbRows = 128;
 
 // Device SELL-C-sigma (layout of src/sell_c_sigma.hpp:43-104, NOTRANS).
 struct DevSell {
@@ -290,6 +310,9 @@ perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
 perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
                                   int32_t C, int32_t sigma, int variant_tag);
 perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A);
+// Column-compressed row blocks of the wide-B SpMM kernel for the CSR plan `d`
+// of operand `o` (sets d.lb_state; a HIP error is returned as is).
+hipError_t build_spmm_blocks(const Operand &o, DevCsr &d);
 // Launch the current plan: y = alpha*A*x + beta*y on device vectors.
 perflibs_status_t launch_plan(perflibs_spmat_top_t *A, double alpha,
                               const void *x, double beta, void *y);

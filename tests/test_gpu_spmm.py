@@ -180,3 +180,93 @@ def test_spmm_long_and_short_rows(pls, orc, k):
         torch.cuda.synchronize()
         ref = orc.spmm(m, k, rp, ci, v, Bv, -1.25, beta, C0 if beta else None)
         assert rel_err(Cd.cpu().numpy(), ref) <= TOL64
+
+
+def banded_with_long_rows(rng, m, n, bw=40, per_row=(0, 30)):
+    """Banded rows (a B row shared by many rows of a block) plus rows with more
+    than 128 distinct columns or 1024 non-zeros, runs of empty rows and
+    duplicated columns."""
+    rows = []
+    for i in range(m):
+        L = int(rng.integers(*per_row))
+        lo, hi = max(0, i - bw), min(n, i + bw + 1)
+        cols = np.sort(rng.choice(np.arange(lo, hi), min(L, hi - lo), replace=False))
+        if i % 211 == 5:
+            cols = np.sort(rng.choice(n, 300, replace=False))       # > 128 distinct columns
+        if i % 503 == 7:
+            cols = np.sort(rng.integers(0, 60, 1500))                 # > 1024 non-zeros, few columns
+        if 900 <= i < 1000:
+            cols = np.array([], np.int64)
+        if i % 37 == 0 and len(cols):
+            cols = np.sort(np.concatenate([cols, cols[:1]]))          # a duplicated column
+        rows.append(cols)
+    rp = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    return rp, ci, rng.standard_normal(int(rp[-1]))
+
+
+@pytest.mark.parametrize("k", [32, 40, 64, 100, 129, 1024])
+def test_spmm_wide_column_compressed_blocks(pls, orc, k):
+    """Wide B through the column-compressed row-block kernel: list blocks,
+    blocks without a list, empty rows, k not a multiple of the column chunk."""
+    import torch
+
+    rng = np.random.default_rng(700 + k)
+    m, n = 3000, 3100
+    rp, ci, v = banded_with_long_rows(rng, m, n)
+    Bv = rng.uniform(-1, 1, (n, k))
+    C0 = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, n, rp, ci, v)
+    Bd = torch.from_numpy(Bv).cuda()
+    for beta in (0.0, 0.5):
+        Cd = torch.from_numpy(C0.copy()).cuda()
+        A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 0.75, beta)
+        torch.cuda.synchronize()
+        ref = orc.spmm(m, k, rp, ci, v, Bv, 0.75, beta, C0 if beta else None)
+        assert rel_err(Cd.cpu().numpy(), ref) <= TOL64
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_spmm_wide_reduced_precision(pls, orc, prec):
+    import torch
+
+    rng = np.random.default_rng(91)
+    m = n = 2500
+    rp, ci, v = banded_with_long_rows(rng, m, n)
+    k = 96
+    v32 = v.astype(np.float32)
+    A = pls.SpMat.csr(m, n, rp, ci, v32)
+    if prec == "bf16":
+        A.set_precision(pls.VALUES_BF16)
+        u = v32.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        stored = (u.astype(np.uint32) << 16).view(np.float32)
+    else:
+        stored = v32
+    Bv = rng.uniform(-1, 1, (n, k)).astype(np.float32)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.zeros((m, k), dtype=torch.float32, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
+    torch.cuda.synchronize()
+    ref = orc.spmm(m, k, rp, ci, stored.astype(np.float64), Bv.astype(np.float64))
+    assert rel_err(Cd.cpu().numpy(), ref) <= 1e-6
+
+
+def test_spmm_k1024_random_columns(pls, orc):
+    """k = 1024 (the reference benchmark's width) on random columns, where the
+    block lists would not shrink B traffic: row-major column slices of 512."""
+    import torch
+
+    rng = np.random.default_rng(92)
+    m, n, k = 1200, 50000, 1024
+    lens = rng.integers(1, 20, m)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(l), replace=False)) for l in lens]).astype(np.int32)
+    v = rng.standard_normal(int(rp[-1]))
+    Bv = rng.uniform(-1, 1, (n, k))
+    A = pls.SpMat.csr(m, n, rp, ci, v)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.zeros((m, k), dtype=torch.float64, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
