@@ -132,7 +132,7 @@ template <class T> perflibs_status_t relayout(perflibs_spmat_top_t *D, int layou
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-// CSR plan of op(A) for SpMM with k columns of B; for k >= 32 also the
+// CSR plan of op(A) for SpMM with k columns of B; for k >= 8 also the
 // column-compressed row blocks of the wide-B kernel (built once per plan).
 perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op, int64_t k) {
   pls::Operand o;
@@ -141,7 +141,7 @@ perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op, int64_t k) {
     const perflibs_status_t st = pls::build_plan(A, o, op, PERFLIBS_AMD_CSR_VECTOR);
     if (st != PERFLIBS_STATUS_SUCCESS) return st;
   }
-  if (k >= 32 && A->plan.csr->lb_state == 0) {
+  if (k >= 8 && A->plan.csr->lb_state == 0) {
     if (!o.prp && !pls::make_operand(A, op, o))
       return pls::exec_fail(A, "matrix format has no GPU SpMM path");
     const hipError_t e = pls::build_spmm_blocks(o, *A->plan.csr);

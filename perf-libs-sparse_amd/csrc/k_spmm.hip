@@ -72,7 +72,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rm(
   }
 }
 
-// Row-block variant (dispatched for k <= 16): one workgroup per row block of the
+// Row-block variant (k < 8, or k <= 16 when the operand has no column-compressed
+// blocks): one workgroup per row block of the
 // CSR plan's nnz-balanced block table (at most kSpmmStage non-zeros and
 // kSpmmRows rows, a longer row alone in its block).  The block's column
 // indices, alpha-scaled values and row offsets are staged into LDS with
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
   }
 }
 
-// Column-compressed row blocks (dispatched for k >= 32 when the plan's block
+// Column-compressed row blocks (dispatched for k >= 8 when the plan's block
 // lists are short enough, see build_spmm_blocks): one workgroup per block of
 // consecutive rows.  The block's alpha-scaled values and one-byte list
 // positions are staged in LDS once; then, KC columns at a time, every B row the
@@ -317,7 +318,8 @@ namespace {
 template <int G, int CPL, class V, class X>
 void launch_rm(const pls::DevCsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
                double beta, void *C, int64_t ldc, hipStream_t s) {
-  // Narrow B (k <= 16): the row-block kernel over the plan's block table.
+  // Narrow B (k <= 16; k >= 8 only without column-compressed blocks): the
+  // row-block kernel over the plan's block table.
   // Wider B rows keep one row per lane group and far more rows in flight
   // (measured faster for k = 32 and 128 on the 27-point stencil).
   if (A.nblocks > 0 && G * CPL <= 16) {
@@ -361,12 +363,16 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
-  // FP64: lists of up to 256 columns in 128-byte row chunks (16 columns);
-  // FP32: lists of up to 128 columns in chunks of 32 (k <= 32) or 64 columns.
-  // At most 32 KB of staged B; the best measured shapes on the 27-point
-  // stencil for each type.
-  if constexpr (sizeof(X) == 8)
+  // Chunks of 8 columns for k <= 8; FP64: lists of up to 256 columns in
+  // 16-column (128-byte) chunks; FP32: lists of up to 128 columns in chunks of
+  // 16, 32 or 64 columns.  At most 32 KB of staged B; the best measured shapes
+  // on the 27-point stencil for each type.
+  if (k <= 8)
+    go(std::integral_constant<int, 8>{}, std::integral_constant<int, sizeof(X) == 8 ? pls::kLbUcols64 : pls::kLbUcols32>{});
+  else if constexpr (sizeof(X) == 8)
     go(std::integral_constant<int, 16>{}, std::integral_constant<int, pls::kLbUcols64>{});
+  else if (k <= 16)
+    go(std::integral_constant<int, 16>{}, std::integral_constant<int, pls::kLbUcols32>{});
   else if (k <= 32)
     go(std::integral_constant<int, 32>{}, std::integral_constant<int, pls::kLbUcols32>{});
   else
@@ -399,7 +405,7 @@ hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha, con
   auto run = [&](auto vtag, auto xtag) -> hipError_t {
     using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
     using X = std::remove_pointer_t<decltype(xtag)>;
-    if (b_sc == 1 && c_sc == 1 && k >= 32 && A.lb_state == 1 && A.lb_nblocks > 0) {
+    if (b_sc == 1 && c_sc == 1 && k >= 8 && A.lb_state == 1 && A.lb_nblocks > 0) {
       launch_lds<V, X>(A, k, alpha, B, b_sr, beta, C, c_sr, s);
     } else if (b_sc == 1 && c_sc == 1) {
       // row-major: column slices of at most 512 (the widest lane-group shape)

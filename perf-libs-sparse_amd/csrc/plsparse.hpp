@@ -108,7 +108,7 @@ struct DevCsr {
   int32_t nblocks = 0;
   int32_t vec_lanes = 8;  // lanes per row of the CSR-vector kernel
   // Column-compressed row blocks of the wide-B SpMM kernel (built on the first
-  // SpMM with k >= 32): each block of consecutive rows lists its distinct
+  // SpMM with k >= 8): each block of consecutive rows lists its distinct
   // columns once (ucol), and each non-zero stores its column's position in
   // that list (lidx, one byte), so a workgroup loads every B row its block
   // needs into LDS once per column chunk.  A block whose row alone exceeds the
