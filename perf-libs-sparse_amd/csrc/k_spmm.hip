@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 // block without a list (one row with more than UMAX distinct columns or
 // kLbNnz non-zeros) spreads its non-zeros over the workgroup's lane groups,
 // reads B directly and sums the groups through LDS.
-template <int KC, int UMAX, class V, class X, bool BETA>
+template <int KC, int UMAX, int CPL, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
     const int32_t *__restrict__ ucol, const uint8_t *__restrict__ lidx, int32_t k,
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     acc_t<X> beta) {
   using A = acc_t<X>;
   constexpr int NG = kBlock / KC;  // lane groups (rows in flight) per workgroup
-  __shared__ A sB[UMAX * KC];
+  __shared__ __align__(16) A sB[UMAX * KC];
   __shared__ A s_a[pls::kLbNnz];
   __shared__ uint8_t s_l[pls::kLbNnz];
   __shared__ int32_t s_rp[pls::kLbRows + 1];
@@ -233,9 +233,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   for (int32_t q = tid; q <= nrows; q += kBlock) s_rp[q] = rp[r0 + q] - e0;
   for (int32_t q = tid; q < nu; q += kBlock) s_uc[q] = ucol[u0 + q];
   __syncthreads();
-  int S = 1;  // groups per row: all groups busy, the S groups of a row in one wave
-  while (S * 2 * KC <= 64 && nrows * S * 2 <= NG) S *= 2;
-  const int sub = (tid % (KC * S)) / KC;
+  // compute mapping: G = KC / CPL lanes per row group, CPL adjacent columns
+  // per lane (read from LDS as one vector); S groups of one wave share a row
+  // when the block has few (long) rows, each taking every S-th non-zero, and
+  // xor shuffles fold them
+  constexpr int G = KC / CPL, NGC = kBlock / G;
+  int S = 1;
+  while (S * 2 * G <= 64 && nrows * S * 2 <= NGC) S *= 2;
+  const int lc = tid % G, sub = (tid % (G * S)) / G;
   for (int32_t c0 = 0; c0 < k; c0 += KC) {
     const int32_t col = c0 + lane;
     // B rows of the list, columns [c0, c0 + KC): NG rows per pass, 4 passes
@@ -252,11 +257,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
         if (i0 + s * NG < nu) sB[(i0 + s * NG) * KC + lane] = t[s];
     }
     __syncthreads();
-    // S lane groups of one wave share a row when the block has few rows
-    // (long rows), each taking every S-th non-zero; xor shuffles fold them
-    for (int32_t rr = tid / (KC * S); rr < nrows; rr += kBlock / (KC * S)) {
+    for (int32_t rr = tid / (G * S); rr < nrows; rr += kBlock / (G * S)) {
       const int32_t p0 = s_rp[rr], p1 = s_rp[rr + 1];
-      A acc = 0;
+      A acc[CPL];
+#pragma unroll
+      for (int t = 0; t < CPL; ++t) acc[t] = 0;
       for (int32_t p = p0 + sub; p < p1; p += 8 * S) {
         A a[8];
         int32_t l[8];
@@ -267,14 +272,32 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
           l[u] = s_l[pp];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += a[u] * sB[l[u] * KC + lane];
+        for (int u = 0; u < 8; ++u) {
+          const A *b = &sB[l[u] * KC + lc * CPL];
+          if constexpr (CPL == 2) {
+            using A2 = std::conditional_t<sizeof(A) == 8, double2, float2>;
+            const A2 v = *reinterpret_cast<const A2 *>(b);
+            acc[0] += a[u] * v.x;
+            acc[1] += a[u] * v.y;
+          } else {
+            acc[0] += a[u] * b[0];
+          }
+        }
       }
-      for (int o = KC; o < KC * S; o <<= 1) acc += __shfl_xor(acc, o);
-      if (sub == 0 && col < k) {
-        X *c = Cm + int64_t(r0 + rr) * ldc + col;
-        A r = acc;
-        if (BETA) r += beta * A(*c);
-        *c = X(r);
+#pragma unroll
+      for (int t = 0; t < CPL; ++t)
+        for (int o = G; o < G * S; o <<= 1) acc[t] += __shfl_xor(acc[t], o);
+      if (sub == 0) {
+        X *c = Cm + int64_t(r0 + rr) * ldc;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+          const int32_t cc = c0 + lc * CPL + t;
+          if (cc < k) {
+            A r = acc[t];
+            if (BETA) r += beta * A(c[cc]);
+            c[cc] = X(r);
+          }
+        }
       }
     }
     __syncthreads();
@@ -355,11 +378,11 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
     const auto *ucol = A.lb_ucol.as<int32_t>();
     const auto *lidx = A.lb_lidx.as<uint8_t>();
     if (beta != 0.0)
-      k_spmm_lds<KC, UMAX, V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
+      k_spmm_lds<KC, UMAX, (KC >= 16 ? 2 : 1), V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
     else
-      k_spmm_lds<KC, UMAX, V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
+      k_spmm_lds<KC, UMAX, (KC >= 16 ? 2 : 1), V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
