@@ -274,11 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const A *b = &sB[l[u] * KC + lc * CPL];
-          if constexpr (CPL == 2) {
+          if constexpr (CPL >= 2) {
             using A2 = std::conditional_t<sizeof(A) == 8, double2, float2>;
-            const A2 v = *reinterpret_cast<const A2 *>(b);
-            acc[0] += a[u] * v.x;
-            acc[1] += a[u] * v.y;
+#pragma unroll
+            for (int h = 0; h < CPL / 2; ++h) {
+              const A2 v = reinterpret_cast<const A2 *>(b)[h];
+              acc[2 * h] += a[u] * v.x;
+              acc[2 * h + 1] += a[u] * v.y;
+            }
           } else {
             acc[0] += a[u] * b[0];
           }
@@ -378,11 +381,11 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
     const auto *ucol = A.lb_ucol.as<int32_t>();
     const auto *lidx = A.lb_lidx.as<uint8_t>();
     if (beta != 0.0)
-      k_spmm_lds<KC, UMAX, (KC >= 16 ? 2 : 1), V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
+      k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
     else
-      k_spmm_lds<KC, UMAX, (KC >= 16 ? 2 : 1), V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
+      k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
