@@ -178,13 +178,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 // consecutive rows.  The block's alpha-scaled values and one-byte list
 // positions are staged in LDS once; then, KC columns at a time, every B row the
 // block needs is loaded into LDS once (coalesced, KC contiguous elements per
-// row) and each row of C is accumulated from LDS by a group of KC lanes (one
-// column each), 8 non-zeros in flight per lane.  On a stencil a B row feeds
-// about 2.7 non-zeros of the block, so L2 -> CU traffic for B falls by that
-// factor against the direct kernels, which fetch a B row per non-zero.  A
-// block without a list (one row with more than UMAX distinct columns or
-// kLbNnz non-zeros) spreads its non-zeros over the workgroup's lane groups,
-// reads B directly and sums the groups through LDS.
+// row) and each row of C is accumulated from LDS by a group of KC / CPL lanes
+// (CPL adjacent columns each), 8 non-zeros in flight per lane.  On a stencil
+// a B row feeds about 2.7 non-zeros of the block, so L2 -> CU traffic for B
+// falls by that factor against the direct kernels, which fetch a B row per
+// non-zero.  A block without a list (one row with more than UMAX distinct
+// columns or 4 * UMAX non-zeros) spreads its non-zeros over the workgroup's
+// lane groups, reads B directly and sums the groups through LDS.
 template <int KC, int UMAX, int CPL, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   using A = acc_t<X>;
   constexpr int NG = kBlock / KC;  // lane groups (rows in flight) per workgroup
   __shared__ __align__(16) A sB[UMAX * KC];
-  __shared__ A s_a[pls::kLbNnz];
-  __shared__ uint8_t s_l[pls::kLbNnz];
+  __shared__ A s_a[pls::kLbNnzPerUcol * UMAX];
+  __shared__ uint8_t s_l[pls::kLbNnzPerUcol * UMAX];
   __shared__ int32_t s_rp[pls::kLbRows + 1];
   __shared__ int32_t s_uc[UMAX];
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -389,20 +389,22 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
-  // Chunks of 8 columns for k <= 8; FP64: lists of up to 256 columns in
-  // 16-column (128-byte) chunks; FP32: lists of up to 128 columns in chunks of
-  // 16, 32 or 64 columns.  At most 32 KB of staged B; the best measured shapes
-  // on the 27-point stencil for each type.
-  if (k <= 8)
-    go(std::integral_constant<int, 8>{}, std::integral_constant<int, sizeof(X) == 8 ? pls::kLbUcols64 : pls::kLbUcols32>{});
-  else if constexpr (sizeof(X) == 8)
-    go(std::integral_constant<int, 16>{}, std::integral_constant<int, pls::kLbUcols64>{});
-  else if (k <= 16)
-    go(std::integral_constant<int, 16>{}, std::integral_constant<int, pls::kLbUcols32>{});
-  else if (k <= 32)
-    go(std::integral_constant<int, 32>{}, std::integral_constant<int, pls::kLbUcols32>{});
-  else
-    go(std::integral_constant<int, 64>{}, std::integral_constant<int, pls::kLbUcols32>{});
+  // Column chunks: 8 for k <= 8, then 16 (FP64) or 16 / 32 (FP32) columns;
+  // at most 32 KB of staged B (16 KB with lists of 128), the best measured
+  // shapes on the 27-point stencil
+  using S_ = std::integral_constant<int, pls::kLbUcolsSmall>;
+  using L_ = std::integral_constant<int, pls::kLbUcolsLarge>;
+  using C8 = std::integral_constant<int, 8>;
+  using C16 = std::integral_constant<int, 16>;
+  using C32 = std::integral_constant<int, 32>;
+  const bool large = A.lb_umax > pls::kLbUcolsSmall;
+  if (k <= 8) {
+    large ? go(C8{}, L_{}) : go(C8{}, S_{});
+  } else if (sizeof(X) == 8 || k <= 16) {
+    large ? go(C16{}, L_{}) : go(C16{}, S_{});
+  } else {
+    large ? go(C32{}, L_{}) : go(C32{}, S_{});
+  }
 }
 
 template <class V, class X>

@@ -181,14 +181,15 @@ static perflibs_status_t build_csr_plan(perflibs_spmat_top_t *A, const Operand &
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-// Greedy cut of consecutive rows into blocks of at most kLbUcols64 (FP64)
-// or kLbUcols32 distinct columns, kLbNnz non-zeros and kLbRows rows; a row that alone exceeds a
+// Greedy cut of consecutive rows into blocks of at most kLbUcols distinct
+// columns, 4 * kLbUcols non-zeros and kLbRows rows; a row that alone exceeds a
 // limit becomes a block without a column list.  Used only when the lists
 // plus the non-zeros of list-less rows are at most 60% of nnz (a B row
 // loaded into LDS then feeds 1.7 non-zeros or more); otherwise the direct
 // kernels, which read B for every non-zero, are used.
-hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
-  const int32_t kLbUcols = d.vs == VStore::F64 ? kLbUcols64 : kLbUcols32;
+namespace {
+hipError_t cut_spmm_blocks(const Operand &o, DevCsr &d, int32_t kLbUcols) {
+  const int32_t kLbNnz = kLbNnzPerUcol * kLbUcols;
   const int32_t m = o.m;
   const int32_t *rp = o.prp, *ci = o.pci;
   const int32_t base = rp[0];
@@ -247,6 +248,7 @@ hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
   if (rows > 0) close(m);
   d.lb_nblocks = int32_t(row0.size() - 1);
   d.lb_ucols = int64_t(ucol.size());
+  d.lb_umax = kLbUcols;
   if (double(d.lb_ucols + direct) > 0.6 * double(nnz)) {
     d.lb_state = -1;
     return hipSuccess;
@@ -256,6 +258,17 @@ hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
   if (e == hipSuccess) e = d.lb_ucol.upload(ucol.data(), sizeof(int32_t) * std::max<size_t>(ucol.size(), 1));
   if (e == hipSuccess) e = d.lb_lidx.upload(lidx.data(), lidx.size());
   d.lb_state = e == hipSuccess ? 1 : 0;
+  return e;
+}
+}  // namespace
+
+// Lists of at most 128 columns keep the kernel's LDS small enough for six
+// workgroups per CU (measured 1.3x faster than 256 on the 27-point stencil at
+// k = 32 and 128); lists of 256 serve operands whose rows alone have more
+// distinct columns (e.g. 8 x 8 block rows).
+hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
+  hipError_t e = cut_spmm_blocks(o, d, kLbUcolsSmall);
+  if (e == hipSuccess && d.lb_state == -1) e = cut_spmm_blocks(o, d, kLbUcolsLarge);
   return e;
 }
 

@@ -115,6 +115,7 @@ struct DevCsr {
   // limits has no list (ucount 0) and reads B directly.
   int lb_state = 0;  // 0: not built, 1: built and used, -1: built, too little reuse
   int32_t lb_nblocks = 0;
+  int32_t lb_umax = 0;   // list length limit the blocks were cut with
   int64_t lb_ucols = 0;  // total list length (sum of distinct columns per block)
   DevBuf lb_row0;        // int32 [nb+1] first row of each block
   DevBuf lb_uoff;        // int32 [nb+1] offsets into lb_ucol
@@ -122,11 +123,11 @@ struct DevCsr {
   DevBuf lb_lidx;        // uint8 [nnz]
 };
 // Limits of a column-compressed SpMM block (must match k_spmm.hip): distinct
-// columns for FP64 and for FP32/BF16 plans (list positions fit a byte),
-// non-zeros and rows.
-constexpr int32_t kLbUcols64 = 256;
-constexpr int32_t kLbUcols32 = 128;
-constexpr int32_t kLbNnz = 1024;
+// columns per list (128, or 256 when lists of 128 leave too many rows without
+// a list; positions fit a byte), non-zeros (4 per list column) and rows.
+constexpr int32_t kLbUcolsSmall = 128;
+constexpr int32_t kLbUcolsLarge = 256;
+constexpr int32_t kLbNnzPerUcol = 4;
 constexpr int32_t kLbRows = 128;
 
 // Device SELL-C-sigma (layout of src/sell_c_sigma.hpp:43-104, NOTRANS).
