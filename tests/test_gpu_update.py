@@ -201,3 +201,44 @@ def test_bsr_handle_update(pls, orc, layout):
     y[:] = 0
     A.spmv(x, y)
     assert rel_err(y, D @ x) <= 1e-12
+
+
+@pytest.mark.parametrize("variant", [1, 7, 11, PANEL])
+def test_update_one_based_handle(pls, orc, variant):
+    """Index base 1: updates name 1-based rows and columns; the device plan is
+    patched at the same entries as for base 0."""
+    rng = np.random.default_rng(74 + variant)
+    m, n = 4096, 3072
+    rp, ci, v, dup_rows = matrix_with_duplicates(rng, m, n)
+    x = rng.uniform(-1, 1, n)
+    A = make(pls, m, n, rp + 1, ci + 1, v, np.float64, variant)
+    spmv(A, x, m, 1.0, 0.0, np.zeros(m), np.float64)
+    rows, cols, vals = pick_updates(rng, rp, ci, 100, dup_rows)
+    A.update(rows + 1, cols + 1, vals)
+    assert A.variant() == variant
+    v_new = host_update(rp, ci, v, rows, cols, vals)
+    y = spmv(A, x, m, 1.0, 0.0, np.zeros(m), np.float64)
+    assert rel_err(y, orc.spmv(m, rp, ci, v_new, x)) <= 1e-12
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_update_then_spmm_column_compressed(pls, orc, gen, k):
+    """SpMM plans with column-compressed row blocks share the CSR values, so an
+    in-place update reaches the wide-B kernel too."""
+    import torch
+
+    rp, ci, v = gen.stencil27(16)
+    m = 16 ** 3
+    rng = np.random.default_rng(75)
+    v = v * rng.uniform(0.5, 1.5, len(v))
+    Bv = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.zeros((m, k), dtype=torch.float64, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
+    rows, cols, vals = pick_updates(rng, rp, ci, 300, np.array([], np.int64))
+    A.update(rows, cols, vals)
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
+    torch.cuda.synchronize()
+    v_new = host_update(rp, ci, v, rows, cols, vals)
+    assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v_new, Bv)) <= 1e-12
