@@ -180,3 +180,41 @@ def test_batch_rejects_bad_input(pls, gen):
     Bt = pls.Batch([A], [1])
     with pytest.raises(pls.PerflibsError):
         Bt.exec(1.0, 0.0)  # operands not set
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_multi_column_pieces_and_unaligned_operands(pls, orc, gen, dtype):
+    """Row shapes with cpl = 2 and 4 columns per lane, on X_p that are 16-byte
+    aligned and on X_p viewed one element into a larger buffer, switching
+    between the two through the pointer table of the same batch."""
+    import torch
+
+    probs, B = make(pls, gen, 24, 41, ks=(2, 4, 8, 16), dtype=dtype, empty=False)
+    vals = [v.astype(dtype).astype(np.float64) for (_, _, _, v, _) in probs]
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rng = np.random.default_rng(42)
+    Xh = [rng.uniform(-1, 1, (m, k)).astype(dtype) for (m, _, _, _, k) in probs]
+    aligned = [torch.from_numpy(x).cuda() for x in Xh]
+    # the same values one element into a larger buffer: 8 (FP64) or 4 (FP32) bytes off
+    bufs = [torch.empty(x.size + 1, dtype=tdt, device="cuda") for x in Xh]
+    for b, x in zip(bufs, Xh):
+        b[1:] = torch.from_numpy(x.reshape(-1)).cuda()
+    shifted = [b[1:] for b in bufs]
+    assert all(s.data_ptr() % 16 != 0 for s in shifted)
+    Y = [torch.zeros((m, k), dtype=tdt, device="cuda") for (m, _, _, _, k) in probs]
+    ref = [orc.spmm(m, k, rp, ci, vals[i], Xh[i].astype(np.float64))
+           for i, (m, rp, ci, _, k) in enumerate(probs)]
+    tol = TOL64 if dtype == np.float64 else 1e-6
+    for cpl in (2, 4):
+        # lanes = k / cpl per row (at least one), cpl capped by k
+        lanes = [max(1, (1 << c) // min(cpl, 1 << c)) for c in range(5)]
+        cpls = [min(cpl, 1 << c) for c in range(5)]
+        B.set_shape(1024, lanes, cpls)
+        for X in (aligned, shifted, aligned):
+            B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+            for y in Y:
+                y.zero_()
+            B.exec(1.0, 0.0)
+            torch.cuda.synchronize()
+            for i in range(len(probs)):
+                assert rel_err(Y[i].cpu().numpy(), ref[i]) <= tol, (cpl, i)
