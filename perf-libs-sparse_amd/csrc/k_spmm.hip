@@ -389,16 +389,17 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
           row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
           (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
-  // Column chunks: 8 for k <= 8, then 16 (FP64) or 16 / 32 (FP32) columns;
-  // at most 32 KB of staged B (16 KB with lists of 128), the best measured
-  // shapes on the 27-point stencil
+  // Column chunks: 8 for k <= 8 and for FP64 lists of 256, then 16 (FP64) or
+  // 16 / 32 (FP32) columns; at most 16 KB of staged FP64 B, so six workgroups
+  // share a CU (the best measured shapes on the 27-point stencil and its 8 x 8
+  // block form)
   using S_ = std::integral_constant<int, pls::kLbUcolsSmall>;
   using L_ = std::integral_constant<int, pls::kLbUcolsLarge>;
   using C8 = std::integral_constant<int, 8>;
   using C16 = std::integral_constant<int, 16>;
   using C32 = std::integral_constant<int, 32>;
   const bool large = A.lb_umax > pls::kLbUcolsSmall;
-  if (k <= 8) {
+  if (k <= 8 || (large && sizeof(X) == 8)) {
     large ? go(C8{}, L_{}) : go(C8{}, S_{});
   } else if (sizeof(X) == 8 || k <= 16) {
     large ? go(C16{}, L_{}) : go(C16{}, S_{});

@@ -270,3 +270,21 @@ def test_spmm_k1024_random_columns(pls, orc):
     A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.0, 0.0)
     torch.cuda.synchronize()
     assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
+
+
+def test_spmm_drop_in_k1024_like_the_reference_benchmark(pls, orc, gen):
+    """The reference benchmark's product through the drop-in objects: CSR A,
+    row-major dense B with 1024 columns, null C, spmm_optimize then spmm_exec."""
+    rp, ci, v = gen.stencil27(10)
+    m = 10 ** 3
+    k = 1024
+    rng = np.random.default_rng(93)
+    v = v * rng.uniform(0.5, 1.5, len(v))
+    Bv = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.reshape(-1))
+    Cm = pls.SpMat.null(m, k)
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ONE, A, B, pls.SCALAR_ZERO, Cm)
+    for _ in range(2):
+        pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cm)
+    assert rel_err(Cm.export_dense(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
