@@ -13,7 +13,7 @@ library's optimize() selected.  `value` is algorithmic GB/s of the timed steps:
 (values + column indices + row pointers + x + y, each counted once).
 
 Also reported: every SpMV variant's GB/s on the same matrix (the exhaustive
-judge) and the selected variant's regret, the HIP-event roofline of the
+sweep) and the selected variant's regret, the HIP-event roofline of the
 dominant kernel, and the reference's CPU path (this project's OpenMP port of
 it) timed on the host.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -122,7 +122,7 @@ def cpu_baseline(n, rp, ci, v, x, y_gpu) -> dict:
         h.close()
     best = min(results, key=results.get)
     t = results[best]
-    # the checker: the GPU result against the CPU port's y of the best kernel
+    # parity check: the GPU result against the CPU port's y of the best kernel
     y[:] = 0
     if best == "csr_vanilla":
         L.orc_cpu_csr_vanilla(n, p(rp), p(ci), p(v), p(x), 1.0, 0.0, p(y))

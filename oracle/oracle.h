@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links or calls this
  * code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * load liboracle.so, and only as the checker / the timed CPU baseline.
+ * load liboracle.so, and only as the parity reference / the timed CPU baseline.
  *
  * Contents (each function cites the reference file:line it restates):
  *   - SELL-C-sigma structure build, data scatter, back-conversion and beta

@@ -1,7 +1,7 @@
 """ctypes wrapper of liboracle.so — TEST INFRASTRUCTURE ONLY.
 
 Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
-always as the checker or the timed CPU baseline, never as the product path.
+always as the parity reference or the timed CPU baseline, never as the product path.
 Each wrapped C function cites the reference file:line it restates (oracle/*.c).
 """
 from __future__ import annotations

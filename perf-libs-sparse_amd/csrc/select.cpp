@@ -4,7 +4,7 @@
 // candidates filtered on padding efficiency, then native CSR / vanilla / SELL
 // timed with a Welch t-test early stop) by four modes:
 //   EXHAUSTIVE — time every applicable variant on the GPU, keep the fastest
-//                (the judge of the other modes' regret);
+//                (the reference for the other modes' regret);
 //   LEARNED    — a 12 -> 64 -> 64 -> K MLP on matrix features predicts
 //                log(time / best time) per variant, argmin wins;
 //   HEURISTIC  — fixed rules on the same features (the no-network baseline);
