@@ -213,6 +213,45 @@ def batch_bench(args) -> int:
     return 0
 
 
+def spmm_cpu_baseline(rp, ci, v, k, nbytes) -> dict:
+    """The reference's row-blocked CSR x dense kernel (oracle/cpu.c port of
+    spmm_rowwise_csr_blocked_m, OpenMP) on the same stencil and k, FP64,
+    with its result checked against the GPU product of the same B."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from oracle import oracle as O
+
+    L = O.lib()
+    threads = O.omp_threads()
+    n = len(rp) - 1
+    rng = np.random.default_rng(0x5EED1003)
+    B = rng.uniform(-1, 1, (n, k))
+    Cm = np.zeros((n, k))
+    p = lambda a: a.ctypes.data
+    ts = []
+    for it in range(3 + 5):
+        Cm[:] = 0.0
+        t0 = time.perf_counter()
+        L.orc_cpu_spmm_rowwise(n, k, p(rp), p(ci), p(v), p(B), k, 1.0, 0.0, p(Cm), k)
+        if it >= 3:
+            ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    A = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+    Bd = torch.from_numpy(B).cuda()
+    Cd = torch.zeros((n, k), dtype=torch.float64, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
+    torch.cuda.synchronize()
+    diff = float(np.linalg.norm(Cd.cpu().numpy() - Cm) / np.linalg.norm(Cm))
+    A.destroy()
+    return {
+        "value": round(nbytes / t / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": f"full csr_fp64_k{k} product, 3 warm-up + median of 5", "ms": round(t * 1e3, 3),
+        "gpu_vs_cpu_rel_diff": diff,
+    }
+
+
 def spmm_bench(args) -> int:
     """CSR x dense SpMM: the 27-point stencil on 128^3 (n = 2,097,152) times a
     row-major B of k in {8, 32, 128} columns, FP64 and FP32, and the BSR
@@ -268,12 +307,14 @@ def spmm_bench(args) -> int:
         A.destroy()
         del brp, bci, bv
     best = lines["csr_fp64_k32"]
+    cpu = None if args.no_cpu else spmm_cpu_baseline(rp, ci, v, 32, best["algorithmic_bytes"])
     out = {
         "metric": METRIC, "value": best["GB/s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "spmm_stencil27_128", "rows": n, "nnz": nnz, "value_line": "csr_fp64_k32"},
         "lines": lines,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
