@@ -208,9 +208,43 @@ def batch_bench(args) -> int:
         "config": {"workload": "batched_banded_4096", "problems": len(probs), "nnz": nnz,
                    "rhs": [1, 4, 16], "seed": 0x5EED0004},
         "precisions": results,
+        "cpu_baseline": None if args.no_cpu else batch_cpu_baseline(
+            probs, X64, y64, results["fp64"]["algorithmic_bytes"]),
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batch_cpu_baseline(probs, X64, y64, nbytes) -> dict:
+    """The batch on the host as the reference would run it (it has no batch
+    API): one call of its row-blocked CSR x dense kernel per problem
+    (oracle/cpu.c port, OpenMP inside each call), FP64, 1 warm-up + median of
+    3 passes over all problems; the result is checked against the GPU batch."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    L = O.lib()
+    threads = O.omp_threads()
+    p = lambda a: a.ctypes.data
+    X = [x.cpu().numpy() for x in X64]
+    Y = [np.zeros((m, k)) for (m, _, _, _, k) in probs]
+    ts = []
+    for it in range(4):
+        for y in Y:
+            y[:] = 0.0
+        t0 = time.perf_counter()
+        for i, (m, rp, ci, v, k) in enumerate(probs):
+            L.orc_cpu_spmm_rowwise(m, k, p(rp), p(ci), p(v), p(X[i]), k, 1.0, 0.0, p(Y[i]), k)
+        if it >= 1:
+            ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    ys = np.concatenate([y.ravel() for y in Y])
+    return {
+        "value": round(nbytes / t / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": "the whole FP64 batch, 1 warm-up + median of 3 passes", "ms": round(t * 1e3, 3),
+        "gpu_vs_cpu_rel_diff": float(np.linalg.norm(ys - y64) / np.linalg.norm(ys)),
+    }
 
 
 def spmm_cpu_baseline(rp, ci, v, k, nbytes) -> dict:
