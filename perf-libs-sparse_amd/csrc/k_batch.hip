@@ -20,16 +20,18 @@
 namespace plk {
 
 // A tile's column indices, alpha-scaled values and row offsets are staged in
-// LDS when they fit (at most kBatchStage non-zeros and kBatchRows rows), so
+// LDS when they fit (at most kBatchStage non-zeros and kBatchRows<A> rows), so
 // the per-row loops only wait for the X rows they gather; a tile that does not
 // fit (a row longer than kBatchStage) reads the matrix from HBM directly.
 constexpr int kBatchStage = 2048;
-constexpr int kBatchRows = 1024;
+// Staged rows: 512 for FP64 tiles (26 KB of LDS, so six workgroups share a
+// CU), 1024 for FP32 (20 KB; more resident workgroups measured slower there).
+template <class A> constexpr int kBatchRows = sizeof(A) == 8 ? 512 : 1024;
 
 template <class A> struct BatchLds {
   int32_t ci[kBatchStage];
   A a[kBatchStage];
-  int32_t rp[kBatchRows + 1];
+  int32_t rp[kBatchRows<A> + 1];
 };
 
 template <int CPL, class V, class X, bool BETA>
@@ -54,7 +56,7 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
   const bool products = alpha != A(0);
   const int32_t e0 = rp[tl.r0];
   const int32_t nnz = rp[tl.r1] - e0, nrows = tl.r1 - tl.r0;
-  const bool staged = nnz <= kBatchStage && nrows <= kBatchRows;
+  const bool staged = nnz <= kBatchStage && nrows <= kBatchRows<A>;
   __syncthreads();  // the previous tile is done with the LDS
   if (staged) {
     constexpr int PER = kBatchStage / kBlock;
@@ -123,7 +125,10 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
 }
 
 template <class V, class X, bool BETA>
-__global__ __launch_bounds__(kBlock) void k_batch(
+// At least six waves per SIMD: caps the FP64 kernel at 84 VGPRs (86 without),
+// which with 26 KB of LDS lets six workgroups share a CU (FP64 batch 7.93 ->
+// 7.75 ms); the FP32 kernels use fewer registers anyway.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(
     int32_t ntiles, const BatchTile *__restrict__ tiles, const BatchProblem *__restrict__ probs,
     const int32_t *__restrict__ rp_all, const int32_t *__restrict__ ci_all,
     const V *__restrict__ val_all, const X *const *__restrict__ xs, X *const *__restrict__ ys,
