@@ -425,7 +425,22 @@ static perflibs_status_t build_panel_plan(perflibs_spmat_top_t *A, const Operand
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+static perflibs_status_t build_plan_layout(perflibs_spmat_top_t *A, const Operand &o, int op,
+                                           int variant);
+
 perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op, int variant) {
+  const perflibs_status_t st = build_plan_layout(A, o, op, variant);
+  if (st == PERFLIBS_STATUS_SUCCESS && op != PERFLIBS_SPARSE_OPERATION_NOTRANS &&
+      (A->fmt == Fmt::Csr || A->fmt == Fmt::Scs)) {
+    // the transposed operand's structure, to locate value updates in place
+    A->plan.h_trp.assign(o.prp, o.prp + size_t(o.m) + 1);
+    A->plan.h_tci.assign(o.pci, o.pci + (int64_t(o.prp[o.m]) - o.prp[0]));
+  }
+  return st;
+}
+
+static perflibs_status_t build_plan_layout(perflibs_spmat_top_t *A, const Operand &o, int op,
+                                           int variant) {
   if (variant == PERFLIBS_AMD_CSR_VECTOR || variant == PERFLIBS_AMD_CSR_ADAPTIVE)
     return build_csr_plan(A, o, op, variant);
   if (variant == PERFLIBS_AMD_CSR_PANEL) return build_panel_plan(A, o, op, variant);

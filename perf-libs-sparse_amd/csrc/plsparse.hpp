@@ -182,7 +182,20 @@ struct Plan {
   std::unique_ptr<DevSell> sell;
   std::unique_ptr<DevBsr> bsr;
   std::unique_ptr<DevPanel> panel;
-  void clear() { variant = -1; csr.reset(); sell.reset(); bsr.reset(); panel.reset(); }
+  // 0-based CSR of the transposed operand a TRANS / CONJTRANS plan was built
+  // from, kept to locate updated entries in it (empty for NOTRANS plans)
+  std::vector<int32_t> h_trp, h_tci;
+  void clear() {
+    variant = -1;
+    csr.reset();
+    sell.reset();
+    bsr.reset();
+    panel.reset();
+    h_trp.clear();
+    h_trp.shrink_to_fit();
+    h_tci.clear();
+    h_tci.shrink_to_fit();
+  }
 };
 
 }  // namespace pls
@@ -328,9 +341,10 @@ struct ValueHit {
   int64_t j;
 };
 // Write the updated host values of `hits` into the device plan in place.
-// Returns false when the plan's layout cannot be patched (a transposed
-// operand, or a CSR-family plan of a BSR handle, whose expansion drops zero
-// values); the caller then drops the plan so the next execution rebuilds it.
+// Returns false when the plan's layout cannot be patched (a CSR-family plan
+// of a BSR handle, whose expansion drops zero values, or a transposed plan of
+// a BSR handle); the caller then drops the plan so the next execution
+// rebuilds it.
 bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits);
 
 // ---- selection (select.cpp) ----

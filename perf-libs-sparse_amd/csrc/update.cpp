@@ -7,7 +7,8 @@
 // first (the CSR / BSR search decides which entry an update names); this file
 // then finds where each updated entry lives in the device layout of the plan
 // and overwrites just those values with one small scatter launch:
-//   CSR-vector / CSR-adaptive: the device CSR is the operand itself, slot = j;
+//   CSR-vector / CSR-adaptive: the device CSR is the operand itself, slot =
+//     the entry's position j in it;
 //   SELL-C-sigma: row r sits at permuted position pi = in2permd[r], chunk
 //     pi / C, lane pi % C, and its t-th entry (CSR order) at cs + t*C + lane;
 //   column panel: entries of (tile, panel) segments keep row order, so the
@@ -17,9 +18,13 @@
 //     (r%b)*b + c%b, holding the sum of the row's entries at column c (the
 //     block build sums duplicates);
 //   native BSR: the device values are the handle's, slot = j.
-// Plans of a transposed operand, and CSR-family plans of a BSR handle (whose
-// expansion drops zero values, so an update can change its structure), are
-// not patched: they are rebuilt by the next execution.
+// Plans of a transposed operand (TRANS / CONJTRANS hints) keep that operand's
+// structure, and each hit is first moved to it: row c of the transpose lists
+// the original rows in ascending order, so the entry (r, c) sits at the lower
+// bound of r in that row (plus its rank among duplicate copies).  CSR-family
+// plans of a BSR handle (whose expansion drops zero values, so an update can
+// change its structure) are not patched: they are rebuilt by the next
+// execution.
 #include <algorithm>
 #include <numeric>
 
@@ -75,12 +80,33 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits) {
   Plan &pl = A->plan;
   if (!pl.csr && !pl.sell && !pl.bsr && !pl.panel) return true;  // nothing on the device yet
   if (hits.empty()) return true;
-  if (pl.op != PERFLIBS_SPARSE_OPERATION_NOTRANS) return false;
+  const bool trans = pl.op != PERFLIBS_SPARSE_OPERATION_NOTRANS;
   const bool csr_host = A->fmt == Fmt::Csr || A->fmt == Fmt::Scs;
-  const bool native_bsr = pl.bsr && !pl.bsr->from_csr && A->fmt == Fmt::Bsr;
+  const bool native_bsr = !trans && pl.bsr && !pl.bsr->from_csr && A->fmt == Fmt::Bsr;
   if (!csr_host && !native_bsr) return false;
+  if (trans && pl.h_trp.empty()) return false;
 
   const size_t nh = hits.size();
+  // The operand the plan was built from: the handle's CSR (NOTRANS) or its
+  // transpose, and every hit in operand coordinates (row, column, position).
+  const int32_t *orp = csr_host ? A->csr.rp : nullptr, *oci = csr_host ? A->csr.ci : nullptr;
+  std::vector<ValueHit> oh(hits);
+  if (trans) {
+    orp = pl.h_trp.data();
+    oci = pl.h_tci.data();
+    const int32_t base = A->csr.rp[0];
+    for (size_t h = 0; h < nh; ++h) {
+      const int32_t r = hits[h].r, c = hits[h].c;
+      // the transposed row c lists original rows in ascending order and the
+      // copies of one (r, c) entry in their original order: the hit is the
+      // k-th copy of column c in row r
+      int64_t kth = 0;
+      for (int64_t j = int64_t(A->csr.rp[r]) - base; j < hits[h].j; ++j)
+        kth += A->csr.ci[j] - base == c;
+      const int32_t *lo = std::lower_bound(oci + orp[c], oci + orp[c + 1], r);
+      oh[h] = {c, r, int64_t(lo - oci) + kth};
+    }
+  }
   std::vector<int64_t> slot(nh, -1);
   // value written per hit, in the handle dtype
   std::vector<double> vd;
@@ -97,24 +123,24 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits) {
   VStore vs = VStore::F64;
   void *dvals = nullptr;
   if (pl.csr) {
-    for (size_t h = 0; h < nh; ++h) slot[h] = hits[h].j;
+    for (size_t h = 0; h < nh; ++h) slot[h] = oh[h].j;
     count = pl.csr->nnz;
     vs = pl.csr->vs;
     dvals = pl.csr->vals.get();
   } else if (pl.sell) {
     const DevSell &S = *pl.sell;
-    const int32_t base = A->csr.rp[0];
+    const int32_t base = orp[0];
     for (size_t h = 0; h < nh; ++h) {
-      const int64_t pi = S.h_in2permd[size_t(hits[h].r)];
+      const int64_t pi = S.h_in2permd[size_t(oh[h].r)];
       const int64_t c = pi / S.C, lane = pi % S.C;
-      const int64_t t = hits[h].j - (int64_t(A->csr.rp[hits[h].r]) - base);
+      const int64_t t = oh[h].j - (int64_t(orp[oh[h].r]) - base);
       slot[h] = S.h_cs[size_t(c)] + t * S.C + lane;
     }
     count = S.padded;
     vs = S.vs;
     dvals = S.vals.get();
   } else if (pl.panel) {
-    panel_slots(*pl.panel, A->csr.rp, A->csr.ci, hits, slot);
+    panel_slots(*pl.panel, orp, oci, oh, slot);
     count = int64_t(pl.panel->vals.bytes() / (pl.panel->vs == VStore::F64 ? 8
                                                : pl.panel->vs == VStore::F32 ? 4 : 2));
     vs = pl.panel->vs;
@@ -129,7 +155,7 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits) {
     const int32_t base = A->csr.rp[0];
     const int64_t bb = int64_t(B.b) * B.b;
     for (size_t h = 0; h < nh; ++h) {
-      const int32_t r = hits[h].r, c = hits[h].c, rb = r / B.b, cb = c / B.b;
+      const int32_t orow = oh[h].r, ocol = oh[h].c, rb = orow / B.b, cb = ocol / B.b;
       int64_t k = -1;
       for (int32_t q = B.h_rp[size_t(rb)]; q < B.h_rp[size_t(rb) + 1]; ++q)
         if (B.h_ci[size_t(q)] == cb) {
@@ -137,9 +163,11 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits) {
           break;
         }
       if (k < 0) return false;
-      slot[h] = k * bb + int64_t(r % B.b) * B.b + c % B.b;
-      // the block value is the sum of the row's entries at column c, added
-      // in CSR order from zero as the block build does
+      slot[h] = k * bb + int64_t(orow % B.b) * B.b + ocol % B.b;
+      // the block value is the sum of every stored copy of the entry, added
+      // in the operand's order from zero as the block build does (for a
+      // transposed operand the copies keep the original row's order)
+      const int32_t r = hits[h].r, c = hits[h].c;
       double sd = 0.0;
       float sf = 0.0f;
       for (int64_t j = int64_t(A->csr.rp[r]) - base; j < int64_t(A->csr.rp[r + 1]) - base; ++j)

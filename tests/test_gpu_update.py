@@ -131,24 +131,38 @@ def test_failed_update_keeps_earlier_ones(pls, orc, variant):
     assert rel_err(y, orc.spmv(m, rp, ci, v_new, x)) <= 1e-12
 
 
-def test_transposed_plan_is_rebuilt(pls):
-    rng = np.random.default_rng(72)
+@pytest.mark.parametrize("variant", [0, 1, 8, 10, PANEL])
+def test_transposed_plan_is_patched(pls, variant):
+    """A plan built for op(A) = A^T is patched in place too: each update is
+    moved to the transposed operand (row c, at the position of original row r
+    among its entries, duplicates in their original order)."""
+    rng = np.random.default_rng(72 + variant)
     m, n = 2048, 3072
-    rp, ci, v = powerlawish(rng, m, n, mean=10.0)
-    A = pls.SpMat.csr(m, n, rp, ci, v)
-    A.hint(pls.HINT_SPMV_OPERATION, pls.TRANS)
-    A.force_variant(PANEL)
-    A.optimize()
+    rp, ci, v, dup_rows = matrix_with_duplicates(rng, m, n)
     x = rng.uniform(-1, 1, m)
+
+    def build(vals):
+        A = pls.SpMat.csr(m, n, rp, ci, vals)
+        A.hint(pls.HINT_SPMV_OPERATION, pls.TRANS)
+        A.force_variant(variant)
+        if variant == PANEL:
+            A.force_panel(7, 1)
+        A.optimize()
+        return A
+
+    A = build(v)
     y = np.zeros(n)
     A.spmv(x, y, trans=pls.TRANS)
-    rows, cols, vals = pick_updates(rng, rp, ci, 50, np.array([], np.int64))
+    rows, cols, vals = pick_updates(rng, rp, ci, 120, np.array(dup_rows))
     A.update(rows, cols, vals)
-    assert A.variant() == -1  # not patched: dropped, rebuilt by the next call
+    assert A.variant() == variant, "the transposed plan was dropped instead of patched"
     y[:] = 0
     A.spmv(x, y, trans=pls.TRANS)
-    assert A.variant() == PANEL
     v_new = host_update(rp, ci, v, rows, cols, vals)
+    F = build(v_new)
+    yf = np.zeros(n)
+    F.spmv(x, yf, trans=pls.TRANS)
+    assert np.array_equal(y, yf)
     D = np.zeros((m, n))
     for i in range(m):
         np.add.at(D[i], ci[rp[i]:rp[i + 1]], v_new[rp[i]:rp[i + 1]])
