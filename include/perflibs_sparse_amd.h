@@ -97,6 +97,14 @@ perflibs_status_t perflibs_amd_force_sell(perflibs_spmat_t A, perflibs_int_t C,
  * workgroups of 4 wave tiles per CU (1..16, or 0 for the default). */
 perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t panel_log2,
                                            perflibs_int_t wg_per_cu);
+/* Column-compressed row blocks of the SpMM plan (for tests and tuning):
+ * blocks_state 0 = not built, 1 = built and used for k >= 8, -1 = built but
+ * too little B-row reuse (direct kernels run); list_max = the list length
+ * limit the blocks were cut with; list_total = total list length.  Status 1
+ * when the handle has no CSR plan yet. */
+perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blocks_state,
+                                         perflibs_int_t *nblocks, perflibs_int_t *list_max,
+                                         int64_t *list_total);
 /* Shape of a column-panel plan (for tests and tuning). */
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,

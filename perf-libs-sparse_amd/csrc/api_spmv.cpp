@@ -285,6 +285,18 @@ perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t pa
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blocks_state,
+                                         perflibs_int_t *nblocks, perflibs_int_t *list_max,
+                                         int64_t *list_total) {
+  if (!A || !A->plan.csr) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  const pls::DevCsr &d = *A->plan.csr;
+  if (blocks_state) *blocks_state = d.lb_state;
+  if (nblocks) *nblocks = d.lb_nblocks;
+  if (list_max) *list_max = d.lb_umax;
+  if (list_total) *list_total = d.lb_ucols;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,
                                           perflibs_int_t *rows_per_tile_max,

@@ -102,6 +102,7 @@ _SIGS = {
     "perflibs_amd_force_sell": (_I, [_P, _I, _I]),
     "perflibs_amd_force_panel": (_I, [_P, _I, _I]),
     "perflibs_amd_panel_info": (_I, [_P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_spmm_info": (_I, [_P, _P, _P, _P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
     "perflibs_amd_mlp_num_params": (C.c_size_t, []),
@@ -291,6 +292,13 @@ class SpMat:
 
     def force_panel(self, panel_log2: int = 0, wg_per_cu: int = 0):
         check("force_panel", lib().perflibs_amd_force_panel(self.h, panel_log2, wg_per_cu))
+
+    def spmm_info(self) -> dict:
+        v = [C.c_int32(), C.c_int32(), C.c_int32()]
+        tot = C.c_int64()
+        check("spmm_info", lib().perflibs_amd_spmm_info(self.h, *[C.byref(a) for a in v], C.byref(tot)))
+        return {"blocks_state": v[0].value, "nblocks": v[1].value, "list_max": v[2].value,
+                "list_total": tot.value}
 
     def panel_info(self) -> dict:
         v = [C.c_int32() for _ in range(5)]

@@ -288,3 +288,36 @@ def test_spmm_drop_in_k1024_like_the_reference_benchmark(pls, orc, gen):
     for _ in range(2):
         pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.0, A, B, 0.0, Cm)
     assert rel_err(Cm.export_dense(), orc.spmm(m, k, rp, ci, v, Bv)) <= TOL64
+
+
+def test_spmm_column_compressed_dispatch(pls, gen):
+    """The wide-B blocks are used where B rows are shared (stencil: lists of
+    128 columns; 8 x 8 blocks: lists of 256) and not on random columns; they are
+    built on the first SpMM with k >= 8 only."""
+    import torch
+
+    def run(m, n, rp, ci, v, k):
+        A = pls.SpMat.csr(m, n, rp, ci, v)
+        Bd = torch.zeros((n, k), dtype=torch.float64, device="cuda")
+        Cd = torch.zeros((m, k), dtype=torch.float64, device="cuda")
+        A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
+        torch.cuda.synchronize()
+        return A.spmm_info()
+
+    rp, ci, v = gen.stencil27(16)
+    m = 16 ** 3
+    assert run(m, m, rp, ci, v, 4)["blocks_state"] == 0
+    info = run(m, m, rp, ci, v, 8)
+    assert info["blocks_state"] == 1 and info["list_max"] == 128
+    assert info["list_total"] < 0.6 * len(ci)
+    brp, bci, bv = gen.block_stencil(6, 5, 4, 8, 0x5EED0003)
+    mb = len(brp) - 1
+    crp, cci, cv = gen.bsr_to_csr_rowmajor(brp, bci, bv, 8)
+    info = run(mb * 8, mb * 8, crp, cci, cv, 32)
+    assert info["blocks_state"] == 1 and info["list_max"] == 256
+    rng = np.random.default_rng(94)
+    m, n = 2000, 400000
+    lens = rng.integers(1, 30, m)
+    rrp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    rci = np.concatenate([np.sort(rng.choice(n, int(l), replace=False)) for l in lens]).astype(np.int32)
+    assert run(m, n, rrp, rci, rng.standard_normal(int(rrp[-1])), 32)["blocks_state"] == -1
