@@ -321,3 +321,30 @@ def test_spmm_column_compressed_dispatch(pls, gen):
     rrp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     rci = np.concatenate([np.sort(rng.choice(n, int(l), replace=False)) for l in lens]).astype(np.int32)
     assert run(m, n, rrp, rci, rng.standard_normal(int(rrp[-1])), 32)["blocks_state"] == -1
+
+
+@pytest.mark.parametrize("k", [8, 40])
+def test_spmm_leading_dimensions(pls, orc, gen, k):
+    """Row-major B and C with leading dimensions wider than k (ldb = k + 3,
+    ldc = k + 5) through the column-compressed kernel; the padding columns of C
+    are left untouched."""
+    import torch
+
+    rp, ci, v = gen.stencil27(14)
+    m = 14 ** 3
+    rng = np.random.default_rng(95 + k)
+    v = v * rng.uniform(0.5, 1.5, len(v))
+    ldb, ldc = k + 3, k + 5
+    Bfull = rng.uniform(-1, 1, (m, ldb))
+    C0 = rng.uniform(-1, 1, (m, ldc))
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    Bd = torch.from_numpy(Bfull).cuda()
+    Cd = torch.from_numpy(C0.copy()).cuda()
+    A.spmm_async(k, Bd.data_ptr(), ldb, Cd.data_ptr(), ldc, 1.25, -0.5)
+    torch.cuda.synchronize()
+    got = Cd.cpu().numpy()
+    ref = orc.spmm(m, k, rp, ci, v, np.ascontiguousarray(Bfull[:, :k]), 1.25, -0.5,
+                   np.ascontiguousarray(C0[:, :k]))
+    assert rel_err(got[:, :k], ref) <= TOL64
+    assert np.array_equal(got[:, k:], C0[:, k:])
+    assert A.spmm_info()["blocks_state"] == 1
