@@ -194,6 +194,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     acc_t<X> beta) {
   using A = acc_t<X>;
   constexpr int NG = kBlock / KC;  // lane groups (rows in flight) per workgroup
+  // B rows in flight per thread while staging: up to 8, at most the list rows
+  // a thread stages
+  constexpr int NIF = UMAX / NG < 8 ? UMAX / NG : 8;
   __shared__ __align__(16) A sB[UMAX * KC];
   __shared__ A s_a[pls::kLbNnzPerUcol * UMAX];
   __shared__ uint8_t s_l[pls::kLbNnzPerUcol * UMAX];
@@ -243,17 +246,17 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   const int lc = tid % G, sub = (tid % (G * S)) / G;
   for (int32_t c0 = 0; c0 < k; c0 += KC) {
     const int32_t col = c0 + lane;
-    // B rows of the list, columns [c0, c0 + KC): NG rows per pass, 4 passes
+    // B rows of the list, columns [c0, c0 + KC): NG rows per pass, NIF passes
     // of loads in flight before their LDS stores
-    for (int32_t i0 = grp; i0 < nu; i0 += 4 * NG) {
-      A t[4];
+    for (int32_t i0 = grp; i0 < nu; i0 += NIF * NG) {
+      A t[NIF];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < NIF; ++s) {
         const int32_t i = min(i0 + s * NG, nu - 1);
         t[s] = col < k ? A(B[int64_t(s_uc[i]) * ldb + col]) : A(0);
       }
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < NIF; ++s)
         if (i0 + s * NG < nu) sB[(i0 + s * NG) * KC + lane] = t[s];
     }
     __syncthreads();
