@@ -241,3 +241,17 @@ def test_csc_ingest_matches_transpose(pls, base):
     assert np.array_equal(E, D)
     assert pls.SpMat.csc(m, n, row_indx, col_ptr + 2, vals, raise_on_error=False)[1] == 1
     assert pls.SpMat.csc(-1, n, row_indx, col_ptr, vals, raise_on_error=False)[1] == 1
+
+
+def test_introspection_without_plans(pls):
+    """Plan introspection on a handle that has no device plan yet is a
+    parameter error (status 1), not a crash."""
+    import ctypes as C
+
+    A = pls.SpMat.csr(3, 3, [0, 1, 2, 3], [0, 1, 2], [1.0, 2.0, 3.0])
+    v = [C.c_int32() for _ in range(5)]
+    tot = C.c_int64()
+    lib = pls.lib()
+    assert lib.perflibs_amd_spmm_info(A.h, *[C.byref(x) for x in v[:3]], C.byref(tot)) == 1
+    assert lib.perflibs_amd_panel_info(A.h, *[C.byref(x) for x in v]) == 1
+    assert lib.perflibs_amd_spmm_info(None, None, None, None, None) == 1
