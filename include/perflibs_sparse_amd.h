@@ -115,6 +115,20 @@ perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *pa
  * computed on the GPU from the CSR structure. */
 perflibs_status_t perflibs_amd_features(perflibs_spmat_t A, float *features);
 
+/* Padding efficiency of SELL-C-sigma(C, sigma) for op(A) (the SpMV hint's
+ * operation): beta = non-zeros / padded slots of real rows, and the smallest
+ * and largest per-thread beta over the reference's nnz-division chunk split
+ * into nthreads threads (nthreads <= 0: the device's CU count).  Counterpart
+ * of perflibs_scs::get_beta / get_beta_threads (src/sell_c_sigma.cpp:369-452);
+ * the chunk widths are computed on the GPU from the row pointers when C and
+ * sigma are powers of two (sigma 0 or at most 16384).  optimize() keeps a
+ * SELL-C-sigma variant as a candidate only at beta >= 0.96 and per-thread
+ * beta > 0.85 (the rule of src/matvec.cpp:303-325).  min_beta / max_beta may
+ * be NULL. */
+perflibs_status_t perflibs_amd_sell_beta(perflibs_spmat_t A, perflibs_int_t C,
+                                         perflibs_int_t sigma, perflibs_int_t nthreads,
+                                         double *beta, double *min_beta, double *max_beta);
+
 /* Time every variant of A (warmup + reps launches each, HIP events on the
  * library stream), writing milliseconds per launch; variants that do not
  * apply get a negative time.  Used by the exhaustive mode and to build

@@ -50,6 +50,11 @@ void orc_scs_fill(orc_scs_t *s, const int32_t *row_ptr, const int32_t *col_indx,
                   const double *vals);
 void orc_scs_free(orc_scs_t *s);
 double orc_scs_beta(const orc_scs_t *s, int64_t nnz);
+/* Thread split of the SELL chunks by padded slots, and the per-thread beta. */
+int orc_scs_decomp_nnz(const orc_scs_t *s, int32_t nthreads, int64_t *chk_offset,
+                       int64_t *nchunks_thread, int64_t *thread_min_nnz);
+int orc_scs_beta_threads(const orc_scs_t *s, int32_t nthreads, int64_t nnz,
+                         double *min_beta, double *max_beta);
 /* scs2csr: returns nnz of the zero-dropped CSR; arrays malloc'd, 0-based. */
 int64_t orc_scs2csr(const orc_scs_t *s, int32_t **row_ptr, int32_t **col_indx,
                     double **vals);

@@ -37,6 +37,8 @@ _SIGS = {
     "orc_scs_fill": (None, [C.POINTER(_Scs), _P, _P, _P]),
     "orc_scs_free": (None, [C.POINTER(_Scs)]),
     "orc_scs_beta": (_D, [C.POINTER(_Scs), _L]),
+    "orc_scs_decomp_nnz": (C.c_int, [C.POINTER(_Scs), _I, _P, _P, _P]),
+    "orc_scs_beta_threads": (C.c_int, [C.POINTER(_Scs), _I, _L, C.POINTER(_D), C.POINTER(_D)]),
     "orc_scs2csr": (_L, [C.POINTER(_Scs), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "orc_scs_spmv": (None, [C.POINTER(_Scs), _P, _P, _D, _D]),
     "orc_bsr_check": (C.c_int, [_I, _I, _I, _I, _P]),
@@ -125,6 +127,30 @@ def csr2scs(m, n, row_ptr, col_indx, vals, Cc, sigma, fill=True) -> dict:
             for p in (rpo, cio, vo):
                 libc.free(p)
         return out
+    finally:
+        lib().orc_scs_free(C.byref(s))
+
+
+def scs_beta_threads(m, n, row_ptr, col_indx, Cc, sigma, nthreads) -> dict:
+    """get_beta / get_beta_threads / gen_parallel_decomp (nnz division) of a
+    fresh SELL-C-sigma object (src/sell_c_sigma.cpp:369-452, 493-552)."""
+    rp, ci = _i32(row_ptr), _i32(col_indx)
+    s = _Scs()
+    if lib().orc_csr2scs(m, n, _p(rp), _p(ci), Cc, sigma, C.byref(s)) != 0:
+        raise ValueError("invalid SELL-C-sigma parameters")
+    try:
+        nnz = int(rp[m] - rp[0])
+        chk = np.zeros(nthreads, np.int64)
+        nct = np.zeros(nthreads, np.int64)
+        tmn = np.zeros(nthreads, np.int64)
+        clamped = lib().orc_scs_decomp_nnz(C.byref(s), nthreads, _p(chk), _p(nct), _p(tmn))
+        lo, hi = C.c_double(), C.c_double()
+        lib().orc_scs_beta_threads(C.byref(s), nthreads, nnz, C.byref(lo), C.byref(hi))
+        return dict(clamped=bool(clamped), beta=lib().orc_scs_beta(C.byref(s), nnz), min_beta=lo.value, max_beta=hi.value,
+                    chk_offset=chk, nchunks_thread=nct, thread_min_nnz=tmn,
+                    cl=_arr(s.cl, s.nC, np.int64), cs=_arr(s.cs, s.nC, np.int64),
+                    chunk_min_nnz=_arr(s.chunk_min_nnz, s.nC, np.int32), ntidyup=s.ntidyup,
+                    n_full_chunks=s.n_full_chunks)
     finally:
         lib().orc_scs_free(C.byref(s))
 

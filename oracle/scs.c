@@ -6,6 +6,8 @@
  *   perflibs_scs::gen_data_vectors / write_offset
  *                               src/sell_c_sigma.cpp:564-705
  *   perflibs_scs::get_beta      src/sell_c_sigma.cpp:369-381
+ *   perflibs_scs::get_beta_threads / gen_parallel_decomp (nnz division)
+ *                               src/sell_c_sigma.cpp:389-452, 493-552
  *   scs2csr / scs2csr_get_nnz_vectors
  *                               src/compressed_sparse_rows.cpp:832-936
  *   spmv_scs<T,C> (generic)     src/sell_c_sigma.cpp:719-797
@@ -212,6 +214,102 @@ double orc_scs_beta(const orc_scs_t *s, int64_t nnz) {
   const int64_t par = s->cs[last] + no_tidyup * s->cl[last] * s->C;
   const int64_t tot = par + s->cl[last] * s->ntidyup;
   return (double)nnz / (double)tot;
+}
+
+/* gen_parallel_decomp, strategy perflibs_scs_parallel_nnz_division
+ * (sell_c_sigma.cpp:493-552), on a freshly built object (all per-thread
+ * arrays zero).  64-bit arithmetic throughout; the reference's perflibs_int_t
+ * (int32 in LP64) gives the same values while the padded slot count stays
+ * below 2^31.  Where the reference's split runs past its last thread (its
+ * assert(this_thread < nthreads) at sell_c_sigma.cpp:527 fails; an NDEBUG
+ * build writes past the per-thread arrays) the remaining chunks stay with the
+ * last thread and the function returns 1; otherwise it returns 0. */
+int orc_scs_decomp_nnz(const orc_scs_t *s, int32_t nthreads, int64_t *chk_offset,
+                       int64_t *nchunks_thread, int64_t *thread_min_nnz) {
+  const int64_t nC = s->nC, C = s->C;
+  for (int32_t i = 0; i < nthreads; i++) {
+    chk_offset[i] = 0;
+    nchunks_thread[i] = 0;
+    thread_min_nnz[i] = 0;
+  }
+  const int64_t no_tidyup = s->ntidyup == 0;
+  const int64_t nnz_scs_par = s->cs[nC - 1] + no_tidyup * s->cl[nC - 1] * C;
+  int64_t vals_per_thread = (nnz_scs_par - nthreads + 1) / nthreads;
+  if (vals_per_thread < 1) vals_per_thread = 1;
+  int64_t vals_next_thread = vals_per_thread, vals_this_thread = 0, this_thread = 0;
+  int clamped = 0;
+  chk_offset[0] = 0;
+  for (int64_t i = 0; i < s->n_full_chunks; i++) {
+    if (vals_this_thread >= vals_next_thread && s->cl[i] > 0) {
+      if (this_thread + 1 >= nthreads) {
+        clamped = 1;
+      } else {
+        if (this_thread > 0)
+          chk_offset[this_thread] = nchunks_thread[this_thread - 1] + chk_offset[this_thread - 1];
+        vals_next_thread += nchunks_thread[this_thread] * C + vals_per_thread;
+        this_thread++;
+      }
+    }
+    vals_this_thread += s->cl[i] * C;
+    nchunks_thread[this_thread]++;
+    thread_min_nnz[this_thread] += s->chunk_min_nnz[i];
+  }
+  chk_offset[this_thread] =
+      this_thread == 0 ? 0 : nchunks_thread[this_thread - 1] + chk_offset[this_thread - 1];
+  thread_min_nnz[this_thread] = 0;
+  for (int64_t c = chk_offset[this_thread]; c < nC; ++c)
+    thread_min_nnz[this_thread] += s->chunk_min_nnz[c];
+  for (++this_thread; this_thread < nthreads; ++this_thread) {
+    chk_offset[this_thread] = nC;
+    nchunks_thread[this_thread] = 0;
+    thread_min_nnz[this_thread] = 0;
+  }
+  return clamped;
+}
+
+/* get_beta_threads (sell_c_sigma.cpp:389-452): the smallest and largest
+ * per-thread padding efficiency thread_min_nnz[t] / (slots of thread t) over
+ * the nnz-division split above; a last thread holding no non-zeros forces the
+ * minimum to 0.  Returns what the split returns (1 when it was clamped). */
+int orc_scs_beta_threads(const orc_scs_t *s, int32_t nthreads, int64_t nnz,
+                         double *min_beta, double *max_beta) {
+  if (nthreads == 1) {
+    *min_beta = *max_beta = orc_scs_beta(s, nnz);
+    return 0;
+  }
+  const int64_t nC = s->nC, C = s->C;
+  int64_t *chk = (int64_t *)calloc((size_t)nthreads, sizeof(int64_t));
+  int64_t *nct = (int64_t *)calloc((size_t)nthreads, sizeof(int64_t));
+  int64_t *tmn = (int64_t *)calloc((size_t)nthreads, sizeof(int64_t));
+  const int clamped = orc_scs_decomp_nnz(s, nthreads, chk, nct, tmn);
+  double mn = 1.0, mx = 0.0;
+  int32_t t = 0;
+  for (; t < nthreads - 1 && nC > 1; ++t) {
+    const int64_t sc = chk[t], ec = chk[t + 1];
+    if (ec == nC) break;
+    const int64_t t_nnz = s->cs[ec] - s->cs[sc];
+    if (tmn[t] > 0) {
+      const double r = (double)tmn[t] / (double)t_nnz;
+      if (r < mn) mn = r;
+      if (r > mx) mx = r;
+    }
+  }
+  const int64_t no_tidyup = s->ntidyup == 0;
+  const int64_t nnz_scs = s->cs[nC - 1] + no_tidyup * s->cl[nC - 1] * C + s->cl[nC - 1] * s->ntidyup;
+  const int64_t last_t_nnz = nnz_scs - s->cs[chk[t]];
+  if (tmn[t] > 0) {
+    const double r = (double)tmn[t] / (double)last_t_nnz;
+    if (r < mn) mn = r;
+    if (r > mx) mx = r;
+  } else {
+    mn = 0.0;
+  }
+  *min_beta = mn;
+  *max_beta = mx;
+  free(chk);
+  free(nct);
+  free(tmn);
+  return clamped;
 }
 
 /* scs2csr (compressed_sparse_rows.cpp:865-936): walks permuted rows 0..m-1,

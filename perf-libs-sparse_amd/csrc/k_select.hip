@@ -163,6 +163,84 @@ hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
+// SELL-C-sigma chunk widths from the row lengths alone: the quantities the
+// padding efficiency of csr2scs's layout depends on (src/sell_c_sigma.cpp:
+// 197-225 window sort and widths, 268-273 chunk_min_nnz).  A workgroup holds
+// kSellWidthRows consecutive row lengths in LDS (rows past m are -1, so they
+// sort behind every real row, as the reference's (0, 0) window padding sorts
+// behind real rows), sorts each aligned window of `win` rows descending with
+// a bitonic network, and writes per chunk of C rows the width cl (its first,
+// i.e. longest, row) and chunk_min_nnz (the sum of its real rows).  With
+// SORT = false the chunks are the rows in order and cl is their maximum.
+// win and C are powers of two dividing kSellWidthRows.
+constexpr int kSellWidthRows = 16384;
+constexpr int kSellWidthThreads = 1024;
+
+template <bool SORT>
+__global__ __launch_bounds__(kSellWidthThreads) void k_sell_widths(
+    int32_t m, const int32_t *__restrict__ rp, int32_t C, int32_t win, int32_t *__restrict__ cl,
+    int64_t *__restrict__ cmn) {
+  __shared__ int32_t len[kSellWidthRows];
+  const int64_t r0 = int64_t(blockIdx.x) * kSellWidthRows;
+  for (int i = threadIdx.x; i < kSellWidthRows; i += kSellWidthThreads) {
+    const int64_t r = r0 + i;
+    len[i] = r < m ? rp[r + 1] - rp[r] : -1;
+  }
+  __syncthreads();
+  if (SORT) {
+    for (int k = 2; k <= win; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < kSellWidthRows; i += kSellWidthThreads) {
+          const int p = i ^ j;
+          if (p > i) {
+            // stages below the window build bitonic runs; the last stage of
+            // the window merges every window descending
+            const bool desc = k == win || (i & k) != 0;
+            const int32_t a = len[i], b = len[p];
+            if (desc ? a < b : a > b) {
+              len[i] = b;
+              len[p] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+  }
+  const int nch = kSellWidthRows / C;
+  for (int j = threadIdx.x; j < nch; j += kSellWidthThreads) {
+    const int64_t c = r0 / C + j;
+    if (c * C >= m) continue;
+    int32_t w = SORT ? len[j * C] : 0;
+    int64_t s = 0;
+    for (int r = 0; r < C; ++r) {
+      const int32_t l = len[j * C + r];
+      if (!SORT) w = max(w, l);
+      if (l > 0) s += l;
+    }
+    cl[c] = max(w, 0);
+    cmn[c] = s;
+  }
+}
+
+bool sell_widths_supported(int32_t C, int32_t sigma) {
+  auto pow2 = [](int32_t v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (!pow2(C) || C > 64) return false;
+  return sigma == 0 || (pow2(sigma) && sigma >= C && sigma <= kSellWidthRows);
+}
+
+hipError_t sell_widths(int32_t m, const int32_t *rp_dev, int32_t C, int32_t sigma,
+                       int32_t *cl_dev, int64_t *cmn_dev, hipStream_t s) {
+  if (!sell_widths_supported(C, sigma)) return hipErrorInvalidValue;
+  const unsigned grid = unsigned((int64_t(m) + kSellWidthRows - 1) / kSellWidthRows);
+  if (grid == 0) return hipSuccess;
+  if (sigma > 0)
+    k_sell_widths<true><<<grid, kSellWidthThreads, 0, s>>>(m, rp_dev, C, sigma, cl_dev, cmn_dev);
+  else
+    k_sell_widths<false><<<grid, kSellWidthThreads, 0, s>>>(m, rp_dev, C, C, cl_dev, cmn_dev);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // MLP.  Parameters flat: W1[I0 x H] b1[H] W2[H x H] b2[H] W3[H x K] b3[K],
 // W stored input-major (W[i * out + o]).
 

@@ -137,6 +137,17 @@ bool variant_candidate(const perflibs_spmat_top_t *A, const Operand &o, int v) {
   if (!variant_applicable(A, o, v)) return false;
   if (v == PERFLIBS_AMD_BSR_B4) return o.fill4 >= 0.75f;
   if (v == PERFLIBS_AMD_BSR_B8) return o.fill8 >= 0.75f;
+  // SELL-C-sigma only with little padding, the reference's audition rule
+  // (src/matvec.cpp:303-325)
+  if (is_sell(v)) return o.sell_rule[v - PERFLIBS_AMD_SELL_C32_S0] != 0;
+  return true;
+}
+
+bool sell_variant_shape(int v, int32_t &C, int32_t &sigma) {
+  if (!is_sell(v)) return false;
+  const SellParams p = sell_params(v);
+  C = p.C;
+  sigma = p.sigma;
   return true;
 }
 

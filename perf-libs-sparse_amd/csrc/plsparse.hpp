@@ -277,6 +277,18 @@ struct SellHost {
 // Structure of csr2scs (src/sell_c_sigma.cpp:145-336); 0-based or 1-based CSR.
 bool sell_structure(int32_t m, const int32_t *rp, const int32_t *ci, int32_t C,
                     int32_t sigma, SellHost &s);
+// Padding efficiency of a SELL-C-sigma layout (get_beta / get_beta_threads,
+// src/sell_c_sigma.cpp:369-452) from its chunk widths cl and chunk_min_nnz.
+struct SellBeta {
+  double beta = 0.0, min_beta = 0.0, max_beta = 0.0;
+  bool clamped = false;  // the reference's thread split would run past its last thread
+};
+SellBeta sell_padding_beta(const std::vector<int32_t> &cl, const std::vector<int64_t> &cmn,
+                           int32_t C, int32_t m, int32_t nthreads);
+// The reference keeps a SELL-C-sigma layout as an audition candidate only at
+// beta >= 0.96 with every thread above 0.85 (src/matvec.cpp:303-325).
+constexpr double kSellBetaMin = 0.96;
+constexpr double kSellThreadBetaMin = 0.85;
 // Scatter of gen_data_vectors (src/sell_c_sigma.cpp:584-705) into typed
 // arrays: vals in the handle dtype, offsets at col_bytes bytes each.
 void sell_fill(const SellHost &s, int32_t m, const int32_t *rp, const int32_t *ci,
@@ -310,6 +322,9 @@ struct Operand {
   int32_t base = 0;
   // block fill nnz / (stored blocks * b^2), filled in by compute_features
   mutable float fill4 = -1.f, fill8 = -1.f;
+  // the SELL-C-sigma variants (in enum order) against the reference's padding
+  // rule: 1 passes, 0 fails, -1 not evaluated; filled in by compute_features
+  mutable int8_t sell_rule[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 bool make_operand(const perflibs_spmat_top_t *A, int op, Operand &o);
 // The variant can run on this operand (dimensions).
@@ -319,6 +334,14 @@ bool variant_applicable(const perflibs_spmat_top_t *A, const Operand &o, int var
 // the stored values by at most 4/3 (on MI355X, BSR below that fill was slower
 // than the best other variant in 163 of 165 measured matrices).
 bool variant_candidate(const perflibs_spmat_top_t *A, const Operand &o, int variant);
+// C and sigma of a SELL-C-sigma variant of the registry; false for other variants.
+bool sell_variant_shape(int variant, int32_t &C, int32_t &sigma);
+// beta and per-thread beta of SELL-C-sigma(C, sigma) for operand o, the chunk
+// widths computed on the GPU from the CSR plan's row pointers `rp_dev` when C
+// and sigma are powers of two (sigma <= 16384, or 0), otherwise on the host
+// from the operand; nthreads <= 0 uses the device's CU count.
+perflibs_status_t sell_beta_of(perflibs_spmat_top_t *A, const Operand &o, const int32_t *rp_dev,
+                               int32_t C, int32_t sigma, int32_t nthreads, SellBeta &out);
 perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
                              int variant);
 perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
@@ -397,6 +420,12 @@ hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s
 // or carry equal values.
 hipError_t patch_vals(VStore vs, void *vals, const int64_t *slot, const void *v, int64_t n,
                       hipStream_t s);
+// SELL-C-sigma chunk widths cl[nC] and chunk_min_nnz[nC] of an m-row 0-based
+// device CSR (C and sigma powers of two, sigma 0 or in [C, 16384]; else
+// hipErrorInvalidValue).
+bool sell_widths_supported(int32_t C, int32_t sigma);
+hipError_t sell_widths(int32_t m, const int32_t *rp_dev, int32_t C, int32_t sigma,
+                       int32_t *cl_dev, int64_t *cmn_dev, hipStream_t s);
 // Row statistics of a 0-based device CSR for the selector features:
 // out[0..] partial sums reduced on the device, see select.cpp.
 hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s);

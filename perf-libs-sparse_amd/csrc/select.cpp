@@ -130,6 +130,57 @@ perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, fl
   feat[11] = float(st[3] / m);
   o.fill4 = feat[9];
   o.fill8 = feat[10];
+  // the reference's padding rule for every SELL-C-sigma variant of the registry
+  for (int v = PERFLIBS_AMD_SELL_C32_S0; v <= PERFLIBS_AMD_SELL_C64_S16384; ++v) {
+    int32_t C = 0, sigma = 0;
+    sell_variant_shape(v, C, sigma);
+    SellBeta b;
+    const perflibs_status_t bs = sell_beta_of(A, o, csr->rp.as<int32_t>(), C, sigma, 0, b);
+    if (bs != PERFLIBS_STATUS_SUCCESS) return bs;
+    o.sell_rule[v - PERFLIBS_AMD_SELL_C32_S0] =
+        int8_t(b.beta >= kSellBetaMin && b.min_beta > kSellThreadBetaMin ? 1 : 0);
+  }
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t sell_beta_of(perflibs_spmat_top_t *A, const Operand &o, const int32_t *rp_dev,
+                               int32_t C, int32_t sigma, int32_t nthreads, SellBeta &out) {
+  if (C <= 0 || C > 64 || sigma < 0 || sigma % C != 0 || o.m <= 0)
+    return param_fail(A, 2, "SELL-C-sigma needs C in [1, 64] and sigma a multiple of C");
+  if (nthreads <= 0) {
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return hip_fail(A, e, "device CU count");
+    nthreads = std::max(cus, 1);
+  }
+  const int64_t nC = (int64_t(o.m) + C - 1) / C;
+  std::vector<int32_t> cl(size_t(nC), 0);
+  std::vector<int64_t> cmn(size_t(nC), 0);
+  if (rp_dev && plk::sell_widths_supported(C, sigma)) {
+    DevBuf dcl, dcmn;
+    hipError_t e = dcl.alloc(sizeof(int32_t) * size_t(nC));
+    if (e == hipSuccess) e = dcmn.alloc(sizeof(int64_t) * size_t(nC));
+    if (e == hipSuccess)
+      e = plk::sell_widths(o.m, rp_dev, C, sigma, dcl.as<int32_t>(), dcmn.as<int64_t>(), stream());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(cl.data(), dcl.get(), sizeof(int32_t) * size_t(nC), hipMemcpyDeviceToHost,
+                         stream());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(cmn.data(), dcmn.get(), sizeof(int64_t) * size_t(nC), hipMemcpyDeviceToHost,
+                         stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) return hip_fail(A, e, "SELL-C-sigma chunk widths");
+  } else {
+    SellHost sh;
+    if (!sell_structure(o.m, o.prp, o.pci, C, sigma, sh))
+      return param_fail(A, 2, "SELL-C-sigma needs C in [1, 64] and sigma a multiple of C");
+    for (int64_t c = 0; c < nC; ++c) {
+      cl[c] = int32_t(sh.cl[c]);
+      cmn[c] = sh.chunk_min_nnz[c];
+    }
+  }
+  out = sell_padding_beta(cl, cmn, C, o.m, nthreads);
   return PERFLIBS_STATUS_SUCCESS;
 }
 
@@ -289,6 +340,33 @@ perflibs_status_t perflibs_amd_features(perflibs_spmat_t A, float *features) {
   pls::Operand o;
   if (!pls::make_operand(A, A->spmv_op, o)) return pls::exec_fail(A, "no CSR view of this matrix");
   return pls::compute_features(A, o, features);
+}
+
+perflibs_status_t perflibs_amd_sell_beta(perflibs_spmat_t A, perflibs_int_t C, perflibs_int_t sigma,
+                                        perflibs_int_t nthreads, double *beta, double *min_beta,
+                                        double *max_beta) {
+  if (!A || !beta) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  hipError_t e = pls::ensure_device();
+  if (e != hipSuccess) return pls::hip_fail(A, e, "no usable HIP device");
+  pls::Operand o;
+  if (!pls::make_operand(A, A->spmv_op, o)) return pls::exec_fail(A, "no CSR view of this matrix");
+  // the row pointers of the operand on the device (a CSR plan's, or a temporary copy)
+  const int32_t *rp_dev = nullptr;
+  pls::DevBuf tmp;
+  if (A->plan.csr && A->plan.op == A->spmv_op) {
+    rp_dev = A->plan.csr->rp.as<int32_t>();
+  } else if (o.m > 0) {
+    e = tmp.upload(o.prp, sizeof(int32_t) * (size_t(o.m) + 1));
+    if (e != hipSuccess) return pls::hip_fail(A, e, "row pointer upload");
+    rp_dev = tmp.as<int32_t>();
+  }
+  pls::SellBeta b;
+  const perflibs_status_t st = pls::sell_beta_of(A, o, rp_dev, C, sigma, nthreads, b);
+  if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  *beta = b.beta;
+  if (min_beta) *min_beta = b.min_beta;
+  if (max_beta) *max_beta = b.max_beta;
+  return PERFLIBS_STATUS_SUCCESS;
 }
 
 perflibs_status_t perflibs_amd_time_variants(perflibs_spmat_t A, int warmup, int reps, double *ms) {

@@ -141,6 +141,77 @@ bool sell_structure(int32_t m, const int32_t *rp, const int32_t *ci, int32_t C,
   return true;
 }
 
+// Padding efficiency of a SELL-C-sigma layout from its chunk widths and
+// chunk_min_nnz.  beta is get_beta (src/sell_c_sigma.cpp:369-381): non-zeros
+// over the slots of real rows, cl * C per full chunk and cl * ntidyup for a
+// partial last chunk.  The per-thread figures follow get_beta_threads
+// (:389-452) over the nnz-division chunk split of gen_parallel_decomp
+// (:493-552), including its threshold recurrence vals_next += (chunks of the
+// thread) * C + vals_per_thread.  Where that recurrence would open a thread
+// past the last one (the reference asserts there) the remaining chunks stay
+// with the last thread.  64-bit arithmetic throughout.
+SellBeta sell_padding_beta(const std::vector<int32_t> &cl, const std::vector<int64_t> &cmn,
+                           int32_t C, int32_t m, int32_t nthreads) {
+  SellBeta out;
+  const int64_t nC = int64_t(cl.size());
+  if (nC == 0 || C <= 0 || nthreads <= 0) return out;
+  std::vector<int64_t> cs(size_t(nC) + 1, 0);
+  for (int64_t c = 0; c < nC; ++c) cs[c + 1] = cs[c] + int64_t(cl[c]) * C;
+  int64_t nnz = 0;
+  for (int64_t v : cmn) nnz += v;
+  const int64_t ntidyup = m % C, n_full = m / C;
+  const int64_t last_w = cl[nC - 1];
+  const int64_t par = cs[nC - 1] + (ntidyup == 0 ? last_w * C : 0);  // slots of full chunks
+  const int64_t slots = par + last_w * ntidyup;
+  out.beta = double(nnz) / double(slots);
+  if (nthreads == 1) {
+    out.min_beta = out.max_beta = out.beta;
+    return out;
+  }
+  // chunk split: thread t owns chunks [start[t], start[t + 1])
+  std::vector<int64_t> start(size_t(nthreads) + 1, nC), count(size_t(nthreads), 0),
+      tmin(size_t(nthreads), 0);
+  const int64_t per = std::max<int64_t>((par - nthreads + 1) / nthreads, 1);
+  int64_t next = per, acc = 0;
+  int32_t t = 0;
+  start[0] = 0;
+  for (int64_t c = 0; c < n_full; ++c) {
+    if (acc >= next && cl[c] > 0) {
+      if (t + 1 < nthreads) {
+        next += count[t] * C + per;
+        start[++t] = c;
+      } else {
+        out.clamped = true;
+      }
+    }
+    acc += int64_t(cl[c]) * C;
+    ++count[t];
+  }
+  // the last thread with chunks also takes the partial last chunk
+  for (int32_t u = 0; u <= t; ++u) {
+    const int64_t hi = u < t ? start[u + 1] : nC;
+    for (int64_t c = start[u]; c < hi; ++c) tmin[u] += cmn[c];
+  }
+  double lo = 1.0, hi = 0.0;
+  for (int32_t u = 0; u < t && nC > 1; ++u) {
+    if (tmin[u] > 0) {
+      const double r = double(tmin[u]) / double(cs[start[u + 1]] - cs[start[u]]);
+      lo = std::min(lo, r);
+      hi = std::max(hi, r);
+    }
+  }
+  if (tmin[t] > 0) {
+    const double r = double(tmin[t]) / double(slots - cs[start[t]]);
+    lo = std::min(lo, r);
+    hi = std::max(hi, r);
+  } else {
+    lo = 0.0;
+  }
+  out.min_beta = lo;
+  out.max_beta = hi;
+  return out;
+}
+
 void sell_fill(const SellHost &s, int32_t m, const int32_t *rp, const int32_t *ci,
                const void *vals, DType dt, std::vector<uint8_t> &vals_out,
                std::vector<uint8_t> &offs_out) {

@@ -12,6 +12,7 @@ library uses in place.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -105,6 +106,7 @@ _SIGS = {
     "perflibs_amd_spmm_info": (_I, [_P, _P, _P, _P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
+    "perflibs_amd_sell_beta": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "perflibs_amd_mlp_num_params": (C.c_size_t, []),
     "perflibs_amd_set_selector_model": (_I, [_P, _P, _P]),
     "perflibs_amd_clear_selector_model": (_I, []),
@@ -139,12 +141,30 @@ def lib() -> C.CDLL:
         except ImportError:
             pass
         handle = C.CDLL(str(LIB_PATH))
+        _check_one_hip_runtime()
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
         _lib = handle
     return _lib
+
+
+def _check_one_hip_runtime() -> None:
+    """Two different libamdhip64 files mapped in one process (PyTorch's and
+    /opt/rocm's) each keep their own device state, and mixing their pointers
+    and streams crashes (a segmentation fault in the first call that passes a
+    torch buffer to this library).  Refuse to continue in that state."""
+    try:
+        maps = Path("/proc/self/maps").read_text()
+    except OSError:
+        return
+    files = {line.split()[-1] for line in maps.splitlines()
+             if "libamdhip64" in line and line.split()[-1].startswith("/")}
+    real = {os.path.realpath(f) for f in files}
+    if len(real) > 1:
+        raise RuntimeError("two HIP runtimes are loaded in this process (" + ", ".join(sorted(real)) +
+                           "): import torch before loading libplsparse_amd, or do not import torch")
 
 
 def check(fn: str, status: int) -> None:
@@ -403,6 +423,13 @@ class SpMat:
         ms = np.zeros(NUM_VARIANTS, dtype=np.float64)
         check("time_variants", lib().perflibs_amd_time_variants(self.h, warmup, reps, _ptr(ms)))
         return ms
+
+    def sell_beta(self, Cc: int, sigma: int, nthreads: int = 0) -> tuple[float, float, float]:
+        """(beta, min per-thread beta, max per-thread beta) of SELL-C-sigma."""
+        b, lo, hi = C.c_double(), C.c_double(), C.c_double()
+        check("sell_beta", lib().perflibs_amd_sell_beta(self.h, Cc, sigma, nthreads, C.byref(b),
+                                                        C.byref(lo), C.byref(hi)))
+        return b.value, lo.value, hi.value
 
     def sell_layout(self) -> dict:
         Cc, sig, cb, nf, nt = (C.c_int32() for _ in range(5))

@@ -323,6 +323,80 @@ def test_spmm_column_compressed_dispatch(pls, gen):
     assert run(m, n, rrp, rci, rng.standard_normal(int(rrp[-1])), 32)["blocks_state"] == -1
 
 
+def block8_operand(gen, long_row=None):
+    """The 8 x 8-block 27-point operator (dense N(0,1) blocks) as CSR; rows of
+    216 distinct columns need the column-compressed lists of 256.  long_row:
+    a row replaced by 300 distinct columns, which no list holds."""
+    brp, bci, bv = gen.block_stencil(6, 5, 4, 8, 0x5EED0003)
+    m = (len(brp) - 1) * 8
+    rp, ci, v = gen.bsr_to_csr_rowmajor(brp, bci, bv, 8)
+    if long_row is not None:
+        rng = np.random.default_rng(7)
+        cols = np.sort(rng.choice(m, 300, replace=False)).astype(np.int32)
+        lens = np.diff(rp)
+        lens[long_row] = 300
+        parts_c = [ci[rp[i]:rp[i + 1]] if i != long_row else cols for i in range(m)]
+        parts_v = [v[rp[i]:rp[i + 1]] if i != long_row else rng.standard_normal(300) for i in range(m)]
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = np.concatenate(parts_c).astype(np.int32)
+        v = np.concatenate(parts_v)
+    return m, brp, bci, bv, rp, ci, v
+
+
+@pytest.mark.parametrize("dtype,k", [(np.float64, 8), (np.float64, 32), (np.float64, 128),
+                                     (np.float32, 8), (np.float32, 16), (np.float32, 40),
+                                     (np.float32, 96)])
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+def test_spmm_lists_of_256_values(pls, orc, gen, dtype, k, beta):
+    """Values of the column-compressed kernel with lists of 256 columns
+    (k_spmm_lds<8|16|32, 256, ...>) on the 8 x 8-block operator, one row
+    without a list: FP64 to 1e-12 against the compensated oracle, FP32 to
+    1e-6 against the exact product of the stored FP32 values."""
+    import torch
+
+    m, _, _, _, rp, ci, v = block8_operand(gen, long_row=333)
+    rng = np.random.default_rng(11 + k)
+    Bv = rng.uniform(-1, 1, (m, k)).astype(dtype)
+    C0 = rng.uniform(-1, 1, (m, k)).astype(dtype)
+    vs = v.astype(dtype)
+    A = pls.SpMat.csr(m, m, rp, ci, vs)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.from_numpy(C0.copy()).cuda()
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.25, beta)
+    torch.cuda.synchronize()
+    info = A.spmm_info()
+    assert info["blocks_state"] == 1 and info["list_max"] == 256
+    ref = orc.spmm(m, k, rp, ci, vs.astype(np.float64), Bv.astype(np.float64), 1.25, beta,
+                   C0.astype(np.float64))
+    tol = TOL64 if dtype == np.float64 else 1e-6
+    assert rel_err(Cd.cpu().numpy(), ref) <= tol
+    # a second call reuses the lists
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 1.25, 0.0)
+    torch.cuda.synchronize()
+    ref0 = orc.spmm(m, k, rp, ci, vs.astype(np.float64), Bv.astype(np.float64), 1.25, 0.0)
+    assert rel_err(Cd.cpu().numpy(), ref0) <= tol
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_spmm_bsr8_drop_in_lists_of_256(pls, orc, gen, k):
+    """The drop-in perflibs_spmm_exec_d on an 8 x 8 BSR handle (expanded like
+    the reference's bsr2csr SpMM path, src/matmul.cpp:619-649) runs the lists
+    of 256 and matches the oracle's bsr2csr + product."""
+    m, brp, bci, bv, _, _, _ = block8_operand(gen)
+    rng = np.random.default_rng(21 + k)
+    Bv = rng.uniform(-1, 1, (m, k))
+    C0 = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.bsr(pls.ROW_MAJOR, m, m, 8, brp, bci, bv)
+    B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.reshape(-1))
+    Cm = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, C0.reshape(-1))
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ANY, Cm)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, -0.75, A, B, 0.5, Cm)
+    rp, ci, v = orc.bsr2csr(91, m, m, 8, brp, bci, bv)
+    assert rel_err(Cm.export_dense(), orc.spmm(m, k, rp, ci, v, Bv, -0.75, 0.5, C0)) <= TOL64
+    info = A.spmm_info()
+    assert info["blocks_state"] == 1 and info["list_max"] == 256
+
+
 @pytest.mark.parametrize("k", [8, 40])
 def test_spmm_leading_dimensions(pls, orc, gen, k):
     """Row-major B and C with leading dimensions wider than k (ldb = k + 3,

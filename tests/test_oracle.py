@@ -285,6 +285,83 @@ def test_scs_brute_force_tiny(orc):
                 assert s["row_permd2in"][:m].tolist() == order
 
 
+def _csr_of_lengths(lens, n=64):
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.arange(l) for l in lens]).astype(np.int32) if sum(lens) else np.zeros(0, np.int32)
+    return rp, ci
+
+
+def test_scs_beta_hand_cases(orc):
+    """get_beta / get_beta_threads on hand-computed padded and tidy-up cases
+    (src/sell_c_sigma.cpp:369-452).  beta counts the padded slots of real rows
+    only: cl[c] * C for full chunks, cl[last] * ntidyup for a partial last
+    chunk; nnz / (cl * C summed over all chunks) would give 12/20 below."""
+    # C = 2, sigma = 0, rows [4,1 | 1,1 | 2,0 | 3,3]: cl = [4,1,2,3], 15 nnz in 20 slots
+    rp, ci = _csr_of_lengths([4, 1, 1, 1, 2, 0, 3, 3])
+    r = orc.scs_beta_threads(8, 64, rp, ci, 2, 0, 2)
+    assert r["cl"].tolist() == [4, 1, 2, 3] and r["ntidyup"] == 0
+    assert r["beta"] == 15 / 20
+    # split: thread 0 = chunks 0-1 (10 slots, 7 nnz), thread 1 = chunks 2-3 (10 slots, 8 nnz)
+    assert r["chk_offset"].tolist() == [0, 2] and r["thread_min_nnz"].tolist() == [7, 8]
+    assert (r["min_beta"], r["max_beta"]) == (7 / 10, 8 / 10)
+    # drop the last row: last chunk holds 1 real row (ntidyup = 1)
+    rp, ci = _csr_of_lengths([4, 1, 1, 1, 2, 0, 3])
+    r = orc.scs_beta_threads(7, 64, rp, ci, 2, 0, 2)
+    assert r["ntidyup"] == 1 and r["n_full_chunks"] == 3
+    assert r["beta"] == 12 / 17  # slots 4*2 + 1*2 + 2*2 + 3*1
+    # vals_per_thread = (14 - 1) // 2 = 6: chunk 0 alone, then chunks 1..3
+    assert r["chk_offset"].tolist() == [0, 1] and r["thread_min_nnz"].tolist() == [5, 7]
+    assert (r["min_beta"], r["max_beta"]) == (5 / 8, 7 / 9)
+    # sigma = 4 sorts windows of 4 rows by (len, idx) descending:
+    # window 0 lens [3,1,0,2] -> chunks {3,2}, {1,0}; window 1 = row 4 alone
+    rp, ci = _csr_of_lengths([3, 1, 0, 2, 4])
+    r = orc.scs_beta_threads(5, 64, rp, ci, 2, 4, 1)
+    assert r["cl"].tolist() == [3, 1, 4] and r["ntidyup"] == 1
+    assert r["beta"] == 10 / 12 and r["min_beta"] == r["max_beta"] == r["beta"]
+    r0 = orc.scs_beta_threads(5, 64, rp, ci, 2, 0, 1)
+    assert r0["cl"].tolist() == [3, 2, 4] and r0["beta"] == 10 / 14
+    # C = 1, a 100-long row first, 3 threads: vals_per_thread = 34; the
+    # thresholds (vals_next += nchunks * C + vals_per_thread) reach 69 and 104
+    # while the slot count is already 101 and 104, so the reference would open
+    # a fourth thread at chunk 3 (its assert, src/sell_c_sigma.cpp:527); the
+    # chunk stays with the last thread
+    rp, ci = _csr_of_lengths([100, 1, 3, 1], n=256)
+    r = orc.scs_beta_threads(4, 256, rp, ci, 1, 0, 3)
+    assert r["clamped"] and r["chk_offset"].tolist() == [0, 1, 2]
+    assert r["nchunks_thread"].tolist() == [1, 1, 2] and r["thread_min_nnz"].tolist() == [100, 1, 4]
+    assert r["beta"] == 1.0 and r["min_beta"] == r["max_beta"] == 1.0
+
+
+@pytest.mark.parametrize("Cc,mult", [(1, 0), (3, 0), (4, 2), (8, 0), (8, 4), (32, 0), (32, 2), (64, 1)])
+def test_scs_beta_brute_force(orc, Cc, mult):
+    """beta = nnz / sum over chunks of cl[c] * (real rows of chunk c), counted
+    directly from the permutation; the thread split partitions the chunks and
+    its per-thread ratios bracket beta."""
+    rng = np.random.default_rng(100 + Cc * 7 + mult)
+    for trial in range(6):
+        m = int(rng.integers(Cc, 40 * Cc + 3))
+        lens = rng.integers(0, 30, m)
+        lens[rng.random(m) < 0.2] = 0
+        lens[rng.random(m) < 0.05] = 200  # long rows make padding
+        rp, ci = _csr_of_lengths(lens.tolist(), n=256)
+        s = orc.csr2scs(m, 256, rp, ci, np.ones(int(lens.sum())), Cc, Cc * mult, fill=False)
+        perm = s["row_permd2in"]
+        slots = sum(int(s["cl"][c]) * int((perm[c * Cc:(c + 1) * Cc] >= 0).sum()) for c in range(s["nC"]))
+        for nth in (1, 2, 5, 16):
+            r = orc.scs_beta_threads(m, 256, rp, ci, Cc, Cc * mult, nth)
+            assert r["beta"] == (lens.sum() / slots if slots else r["beta"])
+            chk, nct, tmn = r["chk_offset"], r["nchunks_thread"], r["thread_min_nnz"]
+            assert chk[0] == 0 and np.all(np.diff(chk) >= 0) and np.all(chk <= s["nC"])
+            assert int(tmn.sum()) == int(lens.sum())
+            for t in range(nth):
+                hi = chk[t + 1] if t + 1 < nth else s["nC"]
+                assert tmn[t] == s["chunk_min_nnz"][chk[t]:hi].sum()
+            if nth == 1:
+                assert r["min_beta"] == r["max_beta"] == r["beta"]
+            elif r["min_beta"] > 0:
+                assert r["min_beta"] <= r["beta"] * (1 + 1e-15) and r["beta"] <= r["max_beta"] * (1 + 1e-15)
+
+
 def test_scs_generic_kernel_matches_oracle(orc):
     rng = np.random.default_rng(9)
     m, n = 500, 400
