@@ -46,10 +46,25 @@ enum perflibs_amd_variant {
 
 /* ---- how perflibs_spmv_optimize picks a variant ---- */
 enum perflibs_amd_select_mode {
-  PERFLIBS_AMD_SELECT_LEARNED = 0,    /* MLP on matrix features (default when a model is loaded) */
-  PERFLIBS_AMD_SELECT_EXHAUSTIVE = 1, /* time every variant, keep the fastest */
+  /* MLP on matrix features ranks the candidates, the best three predictions
+   * are timed (the exhaustive mode's protocol) and the fastest is kept;
+   * features outside the model's training range run the exhaustive sweep.
+   * The default when a model is loaded. */
+  PERFLIBS_AMD_SELECT_LEARNED = 0,
+  PERFLIBS_AMD_SELECT_EXHAUSTIVE = 1, /* time every candidate variant, keep the fastest */
   PERFLIBS_AMD_SELECT_HEURISTIC = 2,  /* fixed rules on the same features, no network */
-  PERFLIBS_AMD_SELECT_FIXED_CSR = 3   /* always PERFLIBS_AMD_CSR_VECTOR */
+  PERFLIBS_AMD_SELECT_FIXED_CSR = 3,  /* always PERFLIBS_AMD_CSR_VECTOR */
+  PERFLIBS_AMD_SELECT_NETWORK = 4     /* the MLP's best prediction alone, nothing timed */
+};
+
+/* How the last perflibs_spmv_optimize of a handle chose its variant. */
+enum perflibs_amd_select_path {
+  PERFLIBS_AMD_PATH_NONE = -1,       /* no selection ran (no optimize, tiny or hinted FEW) */
+  PERFLIBS_AMD_PATH_FIXED = 0,       /* forced variant or fixed CSR */
+  PERFLIBS_AMD_PATH_HEURISTIC = 1,
+  PERFLIBS_AMD_PATH_NETWORK = 2,     /* the network's prediction, nothing timed */
+  PERFLIBS_AMD_PATH_AUDITION = 3,    /* the fastest of the best predictions, timed */
+  PERFLIBS_AMD_PATH_EXHAUSTIVE = 4   /* every candidate timed */
 };
 
 /* ---- value storage precision ---- */
@@ -146,6 +161,15 @@ perflibs_status_t perflibs_amd_set_selector_model(const float *params,
                                                   const float *feat_mean,
                                                   const float *feat_std);
 perflibs_status_t perflibs_amd_clear_selector_model(void);
+/* Training range of each feature (PERFLIBS_AMD_NUM_FEATURES floats each) of
+ * the current model; outside it (beyond a tenth of the range on either side)
+ * the learned mode runs the exhaustive sweep.  NULL arguments forget the
+ * range (then a standardised feature beyond 4 counts as outside).  Status 2
+ * without a model. */
+perflibs_status_t perflibs_amd_set_selector_range(const float *feat_min, const float *feat_max);
+/* Selection path (perflibs_amd_select_path) and number of variants timed by
+ * the last perflibs_spmv_optimize of A; ntimed may be NULL. */
+perflibs_status_t perflibs_amd_select_info(perflibs_spmat_t A, int *path, int *ntimed);
 /* Load a model file written by the training driver (plsparse.selector).  The
  * library loads <dir of libplsparse_amd.so>/../selector/selector_mlp.txt (or the
  * file named by the environment variable PERFLIBS_AMD_SELECTOR_MODEL) by

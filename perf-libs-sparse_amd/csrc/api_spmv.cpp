@@ -90,6 +90,8 @@ perflibs_status_t ensure_plan(perflibs_spmat_top_t *A, int op) {
 // spmv_optimize (src/matvec.cpp:583-630) with the audition replaced by
 // variant selection (learned / exhaustive / heuristic / fixed).
 perflibs_status_t spmv_optimize_impl(perflibs_spmat_top_t *A) {
+  A->select_path = PERFLIBS_AMD_PATH_NONE;
+  A->select_timed = 0;
   if (A->fmt == Fmt::Null || A->fmt == Fmt::Identity || A->fmt == Fmt::Dense)
     return PERFLIBS_STATUS_SUCCESS;
   if (A->fmt == Fmt::Csr && A->nnz == 0) return PERFLIBS_STATUS_SUCCESS;
@@ -256,7 +258,7 @@ perflibs_status_t perflibs_amd_spmv_exec_async_s(enum perflibs_sparse_hint_value
 }
 
 perflibs_status_t perflibs_amd_set_select_mode(perflibs_spmat_t A, int mode) {
-  if (!A || mode < PERFLIBS_AMD_SELECT_LEARNED || mode > PERFLIBS_AMD_SELECT_FIXED_CSR)
+  if (!A || mode < PERFLIBS_AMD_SELECT_LEARNED || mode > PERFLIBS_AMD_SELECT_NETWORK)
     return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   A->select_mode = mode;
   return PERFLIBS_STATUS_SUCCESS;

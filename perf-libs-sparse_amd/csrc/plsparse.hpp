@@ -229,6 +229,8 @@ struct perflibs_spmat_top_t {
 
   pls::Plan plan;      // device plan for SpMV / SpMM
   int select_mode = -1;   // -1: learned when a model is loaded, else heuristic
+  int select_path = -1;   // how the last optimize() chose (perflibs_amd_select_path)
+  int select_timed = 0;   // variants it timed
   int forced_variant = -1;
   int forced_C = -1, forced_sigma = -1;
   int forced_panel_log2 = -1, forced_panel_wpc = -1;  // column-panel shape overrides
@@ -375,8 +377,10 @@ int pick_variant(perflibs_spmat_top_t *A, const Operand &o);
 perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o,
                                    float *feat);
 int heuristic_variant(const float *feat, const Operand &o);
+// Times the candidate variants (only[v] false: skipped, only == nullptr: all)
+// with warmup + reps launches each; ms[v] < 0 for variants not timed.
 perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op,
-                                int warmup, int reps, double *ms);
+                                int warmup, int reps, double *ms, const bool *only);
 
 }  // namespace pls
 

@@ -6,7 +6,13 @@
 //   EXHAUSTIVE — time every applicable variant on the GPU, keep the fastest
 //                (the reference for the other modes' regret);
 //   LEARNED    — a 12 -> 64 -> 64 -> K MLP on matrix features predicts
-//                log(time / best time) per variant, argmin wins;
+//                log(time / best time) per variant; the best three
+//                predictions are timed like the exhaustive mode and the
+//                fastest wins (the reference's timed audition as the safety
+//                net), and features outside the training range fall back to
+//                the exhaustive sweep;
+//   NETWORK    — the MLP's argmin alone, no timing (the ablation of the
+//                audition);
 //   HEURISTIC  — fixed rules on the same features (the no-network baseline);
 //   FIXED_CSR  — always the CSR-vector kernel.
 #include <dlfcn.h>
@@ -35,14 +41,23 @@ struct Model {
   bool default_tried = false;  // the shipped model file was looked for
   DevBuf params;
   float mean[kF] = {0}, stdv[kF] = {0};
+  // range of every feature over the training matrices (has_range = false:
+  // unknown, a standardised feature beyond +-kZmax counts as outside)
+  bool has_range = false;
+  float fmin[kF] = {0}, fmax[kF] = {0};
 };
+constexpr int kAudition = 3;     // predictions timed in learned mode
+constexpr float kZmax = 4.0f;    // without a stored range
+constexpr float kRangeSlack = 0.1f;  // share of a feature's training range accepted beyond it
 Model g_model;
 
 // Model file (text): "plsparse-selector-mlp 1", "features F hidden H outputs K",
 // then lines "mean ...", "std ..." and "params ..." of whitespace-separated
-// floats (the flat parameter layout of perflibs_amd_set_selector_model).
+// floats (the flat parameter layout of perflibs_amd_set_selector_model), and
+// optionally "min ..." and "max ...": the training range of each feature.
 bool parse_model(const std::string &text, std::vector<float> &params, std::vector<float> &mean,
-                 std::vector<float> &stdv, std::string &why) {
+                 std::vector<float> &stdv, std::vector<float> &fmin, std::vector<float> &fmax,
+                 std::string &why) {
   std::istringstream in(text);
   std::string line, magic;
   int version = 0;
@@ -70,6 +85,12 @@ bool parse_model(const std::string &text, std::vector<float> &params, std::vecto
   };
   if (!read_row("mean", mean, kF) || !read_row("std", stdv, kF) || !read_row("params", params, kNumParams))
     return why = "model file rows are missing or have the wrong length", false;
+  // optional: the training range of every feature ("min ...", "max ...")
+  fmin.clear();
+  fmax.clear();
+  if (read_row("min", fmin, kF) && read_row("max", fmax, kF)) return true;
+  fmin.clear();
+  fmax.clear();
   return true;
 }
 
@@ -198,7 +219,7 @@ int heuristic_variant(const float *f, const Operand &o) {
 }
 
 perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op, int warmup,
-                                int reps, double *ms) {
+                                int reps, double *ms, const bool *only) {
   hipError_t e = ensure_device();
   if (e != hipSuccess) return hip_fail(A, e, "no usable HIP device");
   const size_t es = A->dt == DType::F64 ? 8 : 4;
@@ -218,7 +239,7 @@ perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int o
   perflibs_status_t status = PERFLIBS_STATUS_SUCCESS;
   for (int v = 0; v < kK; ++v) {
     ms[v] = -1.0;
-    if (!variant_candidate(A, o, v)) continue;
+    if ((only && !only[v]) || !variant_candidate(A, o, v)) continue;
     perflibs_status_t st = build_plan(A, o, op, v);
     if (st != PERFLIBS_STATUS_SUCCESS) continue;
     for (int w = 0; w < warmup && st == PERFLIBS_STATUS_SUCCESS; ++w)
@@ -242,10 +263,43 @@ perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int o
   return status;
 }
 
-// MLP inference on the GPU: standardised features -> predicted log slowdown.
-static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const float *f) {
+// The features lie inside the training range (widened by kRangeSlack of its
+// width on each side; a feature constant in training accepts +-kRangeSlack),
+// or, for a model without a stored range, within kZmax standard deviations.
+static bool features_in_range(const float *f, const float *xs) {
+  for (int i = 0; i < kF; ++i) {
+    if (!std::isfinite(f[i])) return false;
+    if (g_model.has_range) {
+      const float w = std::max(g_model.fmax[i] - g_model.fmin[i], 1.0f) * kRangeSlack;
+      if (f[i] < g_model.fmin[i] - w || f[i] > g_model.fmax[i] + w) return false;
+    } else if (std::fabs(xs[i]) > kZmax) {
+      return false;
+    }
+  }
+  return true;
+}
+
+static int fastest(const double *ms, int fallback) {
+  int best = -1;
+  for (int v = 0; v < kK; ++v)
+    if (ms[v] > 0 && (best < 0 || ms[v] < ms[best])) best = v;
+  return best < 0 ? fallback : best;
+}
+
+// MLP inference on the GPU: standardised features -> predicted log slowdown;
+// then the audition (audition == true) or the network's argmin alone.
+static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const float *f, bool audition) {
   float xs[kF];
   for (int i = 0; i < kF; ++i) xs[i] = (f[i] - g_model.mean[i]) / (g_model.stdv[i] > 0 ? g_model.stdv[i] : 1.f);
+  if (audition && !features_in_range(f, xs)) {
+    double ms[kK];
+    A->select_path = PERFLIBS_AMD_PATH_EXHAUSTIVE;
+    if (time_variants(A, o, A->spmv_op, 3, 10, ms, nullptr) != PERFLIBS_STATUS_SUCCESS)
+      return PERFLIBS_AMD_CSR_VECTOR;
+    A->select_timed = 0;
+    for (int v = 0; v < kK; ++v) A->select_timed += ms[v] > 0;
+    return fastest(ms, PERFLIBS_AMD_CSR_VECTOR);
+  }
   DevBuf X, h1, h2, out;
   hipError_t e = X.upload(xs, sizeof(xs));
   if (e == hipSuccess) e = h1.alloc(sizeof(float) * kH);
@@ -256,19 +310,35 @@ static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const floa
                          h2.as<float>(), out.as<float>(), stream());
   float pred[kK];
   if (e == hipSuccess) e = hipMemcpy(pred, out.get(), sizeof(pred), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return heuristic_variant(f, o);
-  int best = PERFLIBS_AMD_CSR_VECTOR;
-  float bv = 1e30f;
+  if (e != hipSuccess) {
+    A->select_path = PERFLIBS_AMD_PATH_HEURISTIC;
+    return heuristic_variant(f, o);
+  }
+  int order[kK], nc = 0;
   for (int v = 0; v < kK; ++v)
-    if (variant_candidate(A, o, v) && pred[v] < bv) {
-      bv = pred[v];
-      best = v;
-    }
-  return best;
+    if (variant_candidate(A, o, v)) order[nc++] = v;
+  std::stable_sort(order, order + nc, [&](int a, int b) { return pred[a] < pred[b]; });
+  if (nc == 0) return PERFLIBS_AMD_CSR_VECTOR;
+  A->select_path = PERFLIBS_AMD_PATH_NETWORK;
+  if (!audition || nc == 1) return order[0];
+  // time the best predictions with the exhaustive mode's protocol
+  bool only[kK] = {false};
+  const int na = std::min(nc, kAudition);
+  for (int i = 0; i < na; ++i) only[order[i]] = true;
+  double ms[kK];
+  if (time_variants(A, o, A->spmv_op, 3, 10, ms, only) != PERFLIBS_STATUS_SUCCESS) {
+    (void)hipGetLastError();
+    return order[0];
+  }
+  A->select_path = PERFLIBS_AMD_PATH_AUDITION;
+  A->select_timed = na;
+  return fastest(ms, order[0]);
 }
 
 int pick_variant(perflibs_spmat_top_t *A, const Operand &o) {
   const int op = A->spmv_op;
+  A->select_path = PERFLIBS_AMD_PATH_FIXED;
+  A->select_timed = 0;
   if (A->forced_variant >= 0)
     return variant_applicable(A, o, A->forced_variant) ? A->forced_variant : PERFLIBS_AMD_CSR_VECTOR;
   int mode = A->select_mode;
@@ -277,15 +347,16 @@ int pick_variant(perflibs_spmat_top_t *A, const Operand &o) {
   if (mode == PERFLIBS_AMD_SELECT_FIXED_CSR) return PERFLIBS_AMD_CSR_VECTOR;
   if (mode == PERFLIBS_AMD_SELECT_EXHAUSTIVE) {
     double ms[kK];
-    if (time_variants(A, o, op, 3, 10, ms) != PERFLIBS_STATUS_SUCCESS) return PERFLIBS_AMD_CSR_VECTOR;
-    int best = PERFLIBS_AMD_CSR_VECTOR;
-    for (int v = 0; v < kK; ++v)
-      if (ms[v] > 0 && (ms[best] <= 0 || ms[v] < ms[best])) best = v;
-    return best;
+    A->select_path = PERFLIBS_AMD_PATH_EXHAUSTIVE;
+    if (time_variants(A, o, op, 3, 10, ms, nullptr) != PERFLIBS_STATUS_SUCCESS) return PERFLIBS_AMD_CSR_VECTOR;
+    for (int v = 0; v < kK; ++v) A->select_timed += ms[v] > 0;
+    return fastest(ms, PERFLIBS_AMD_CSR_VECTOR);
   }
   float f[kF];
   if (compute_features(A, o, f) != PERFLIBS_STATUS_SUCCESS) return PERFLIBS_AMD_CSR_VECTOR;
-  if (mode == PERFLIBS_AMD_SELECT_LEARNED && g_model.loaded) return learned_variant(A, o, f);
+  if ((mode == PERFLIBS_AMD_SELECT_LEARNED || mode == PERFLIBS_AMD_SELECT_NETWORK) && g_model.loaded)
+    return learned_variant(A, o, f, mode == PERFLIBS_AMD_SELECT_LEARNED);
+  A->select_path = PERFLIBS_AMD_PATH_HEURISTIC;
   return heuristic_variant(f, o);
 }
 
@@ -303,7 +374,29 @@ perflibs_status_t perflibs_amd_set_selector_model(const float *params, const flo
   if (e != hipSuccess) return pls::hip_fail(nullptr, e, "selector model upload");
   std::memcpy(pls::g_model.mean, feat_mean, sizeof(pls::g_model.mean));
   std::memcpy(pls::g_model.stdv, feat_std, sizeof(pls::g_model.stdv));
+  pls::g_model.has_range = false;
   pls::g_model.loaded = true;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_set_selector_range(const float *feat_min, const float *feat_max) {
+  if (!pls::g_model.loaded) return PERFLIBS_STATUS_EXECUTION_FAILURE;
+  if (!feat_min || !feat_max) {
+    pls::g_model.has_range = false;
+    return PERFLIBS_STATUS_SUCCESS;
+  }
+  for (int i = 0; i < pls::kF; ++i)
+    if (!(feat_min[i] <= feat_max[i])) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  std::memcpy(pls::g_model.fmin, feat_min, sizeof(pls::g_model.fmin));
+  std::memcpy(pls::g_model.fmax, feat_max, sizeof(pls::g_model.fmax));
+  pls::g_model.has_range = true;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_select_info(perflibs_spmat_t A, int *path, int *ntimed) {
+  if (!A || !path) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  *path = A->select_path;
+  if (ntimed) *ntimed = A->select_timed;
   return PERFLIBS_STATUS_SUCCESS;
 }
 
@@ -316,14 +409,17 @@ perflibs_status_t perflibs_amd_load_selector_model(const char *path) {
   }
   std::stringstream ss;
   ss << f.rdbuf();
-  std::vector<float> params, mean, stdv;
+  std::vector<float> params, mean, stdv, fmin, fmax;
   std::string why;
-  if (!pls::parse_model(ss.str(), params, mean, stdv, why)) {
+  if (!pls::parse_model(ss.str(), params, mean, stdv, fmin, fmax, why)) {
     pls::set_last_error(std::string("selector model ") + path + ": " + why);
     return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   }
   pls::g_model.default_tried = true;
-  return perflibs_amd_set_selector_model(params.data(), mean.data(), stdv.data());
+  perflibs_status_t st = perflibs_amd_set_selector_model(params.data(), mean.data(), stdv.data());
+  if (st == PERFLIBS_STATUS_SUCCESS && !fmin.empty())
+    st = perflibs_amd_set_selector_range(fmin.data(), fmax.data());
+  return st;
 }
 
 perflibs_status_t perflibs_amd_clear_selector_model(void) {
@@ -373,7 +469,7 @@ perflibs_status_t perflibs_amd_time_variants(perflibs_spmat_t A, int warmup, int
   if (!A || !ms || reps <= 0 || warmup < 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   pls::Operand o;
   if (!pls::make_operand(A, A->spmv_op, o)) return pls::exec_fail(A, "no CSR view of this matrix");
-  return pls::time_variants(A, o, A->spmv_op, warmup, reps, ms);
+  return pls::time_variants(A, o, A->spmv_op, warmup, reps, ms, nullptr);
 }
 
 // ---- MLP primitives (device buffers) ----

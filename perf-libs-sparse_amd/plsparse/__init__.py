@@ -41,7 +41,8 @@ VARIANTS = [
 NUM_VARIANTS = len(VARIANTS)
 NUM_FEATURES = 12
 MLP_HIDDEN = 64
-SELECT_LEARNED, SELECT_EXHAUSTIVE, SELECT_HEURISTIC, SELECT_FIXED_CSR = 0, 1, 2, 3
+SELECT_LEARNED, SELECT_EXHAUSTIVE, SELECT_HEURISTIC, SELECT_FIXED_CSR, SELECT_NETWORK = 0, 1, 2, 3, 4
+PATH_NONE, PATH_FIXED, PATH_HEURISTIC, PATH_NETWORK, PATH_AUDITION, PATH_EXHAUSTIVE = -1, 0, 1, 2, 3, 4
 VALUES_NATIVE, VALUES_BF16 = 0, 2
 
 
@@ -111,6 +112,8 @@ _SIGS = {
     "perflibs_amd_set_selector_model": (_I, [_P, _P, _P]),
     "perflibs_amd_clear_selector_model": (_I, []),
     "perflibs_amd_load_selector_model": (_I, [C.c_char_p]),
+    "perflibs_amd_set_selector_range": (_I, [_P, _P]),
+    "perflibs_amd_select_info": (_I, [_P, _P, _P]),
     "perflibs_amd_mlp_forward": (_I, [_P, _P, C.c_int, _P, _P, _P]),
     "perflibs_amd_mlp_backward": (_I, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_mlp_adam": (_I, [_P, _P, _P, _P, C.c_int, _F, _F, _F, _F]),
@@ -336,6 +339,12 @@ class SpMat:
         v = C.c_int()
         check("get_variant", lib().perflibs_amd_get_variant(self.h, C.byref(v)))
         return v.value
+
+    def select_info(self) -> tuple[int, int]:
+        """(selection path of the last optimize(), number of variants it timed)."""
+        p, t = C.c_int(), C.c_int()
+        check("select_info", lib().perflibs_amd_select_info(self.h, C.byref(p), C.byref(t)))
+        return p.value, t.value
 
     # ---- execution ----
     def spmv(self, x, y, alpha=1.0, beta=0.0, trans=NOTRANS, raise_on_error=True) -> int:

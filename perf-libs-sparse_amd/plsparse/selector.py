@@ -237,21 +237,29 @@ def picks_of(pred: np.ndarray, times: np.ndarray) -> np.ndarray:
     return np.argmin(np.where(times > 0, pred, np.inf), axis=1)
 
 
-def save_model(path: Path, params, mean, std):
+def save_model(path: Path, params, mean, std, fmin=None, fmax=None):
+    """Model file; fmin / fmax = the training range of each feature (the
+    library's learned mode runs the exhaustive sweep outside it)."""
     lines = ["plsparse-selector-mlp 1",
              f"features {NUM_FEATURES} hidden {HIDDEN} outputs {NUM_VARIANTS}",
              "mean " + " ".join(f"{x:.9g}" for x in mean),
              "std " + " ".join(f"{x:.9g}" for x in std),
              "params " + " ".join(f"{x:.9g}" for x in params)]
+    if fmin is not None and fmax is not None:
+        lines += ["min " + " ".join(f"{x:.9g}" for x in fmin),
+                  "max " + " ".join(f"{x:.9g}" for x in fmax)]
     Path(path).write_text("\n".join(lines) + "\n")
 
 
-def load_model(path: Path):
+def load_model(path: Path, with_range=False):
     rows = {}
     for line in Path(path).read_text().splitlines():
         key, _, rest = line.partition(" ")
         rows[key] = rest
     arr = lambda k: np.array([float(t) for t in rows[k].split()], np.float32)
+    if with_range:
+        rng = (arr("min"), arr("max")) if "min" in rows and "max" in rows else (None, None)
+        return arr("params"), arr("mean"), arr("std"), rng[0], rng[1]
     return arr("params"), arr("mean"), arr("std")
 
 
@@ -291,7 +299,7 @@ def main(argv=None) -> int:
             val = (feats[vsel], times[vsel])
             feats, times = feats[~vsel], times[~vsel]
         params, mean, std = train(feats, times, steps=args.steps, log=log, val=val)
-        save_model(Path(args.model), params, mean, std)
+        save_model(Path(args.model), params, mean, std, feats.min(axis=0), feats.max(axis=0))
         pred = predict(params, mean, std, feats)
         log(f"[train] training-set {score(picks_of(pred, times), times)}")
     else:

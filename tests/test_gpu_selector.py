@@ -96,17 +96,39 @@ def test_features_of_known_matrices(pls, gen):
     assert abs(E.features()[11] - 6 / 8) < 1e-6
 
 
-def test_time_variants_and_exhaustive_mode(pls, gen):
+def cu_count():
+    import torch
+
+    return torch.cuda.get_device_properties(0).multi_processor_count
+
+
+SELL_SHAPES = [(32, 0), (32, 64), (32, 1024), (32, 16384), (64, 0), (64, 64), (64, 1024), (64, 16384)]
+
+
+def sell_rule(orc, m, n, rp, ci):
+    """The reference's SELL audition rule per registry variant (beta >= 0.96,
+    every thread above 0.85; src/matvec.cpp:303-325), threads = CUs."""
+    out = []
+    for Cc, sigma in SELL_SHAPES:
+        r = orc.scs_beta_threads(m, n, rp, ci, Cc, sigma, cu_count())
+        out.append(r["beta"] >= 0.96 and r["min_beta"] > 0.85)
+    return np.array(out)
+
+
+def test_time_variants_and_exhaustive_mode(pls, orc, gen):
     rp, ci, v = gen.powerlaw(1 << 16, 11)
     n = 1 << 16
     A = pls.SpMat.csr(n, n, rp, ci, v)
     ms = A.time_variants(2, 5)
-    assert np.all(ms[:10] > 0) and ms[12] > 0
+    assert np.all(ms[:2] > 0) and ms[12] > 0
+    # SELL layouts only where the padding rule passes
+    assert np.array_equal(ms[2:10] > 0, sell_rule(orc, n, n, rp, ci))
     # random columns: 4x4 / 8x8 blocks would be ~1/16 full, so BSR is not a candidate
     assert np.all(ms[10:12] < 0)
     A.select_mode(pls.SELECT_EXHAUSTIVE)
     A.optimize()
-    assert 0 <= A.variant() < K
+    assert 0 <= A.variant() < K and ms[A.variant()] > 0
+    assert A.select_info() == (pls.PATH_EXHAUSTIVE, int((ms > 0).sum()))
     ref = np.zeros(n)
     x = np.linspace(-1, 1, n)
     A.spmv(x, ref)
@@ -120,27 +142,48 @@ def test_time_variants_and_exhaustive_mode(pls, gen):
 
 
 def test_learned_mode_uses_model(pls, gen):
+    """A model whose output biases rank variant 4, then 7, then 2 first: the
+    network mode takes 4; the learned mode times exactly those three and keeps
+    the fastest (checked against the times it measured: the pick is one of
+    them); features outside the stored training range run the exhaustive
+    sweep."""
     rng = np.random.default_rng(3)
     P = (rng.standard_normal(mlp.num_params(K)) * 0.1).astype(np.float32)
-    # bias the output layer so variant 4 is predicted fastest
+    P[: -K] *= 1e-3  # hidden layers near zero: the output biases decide
     b3 = np.zeros(K, np.float32) + 5.0
-    b3[4] = -5.0
+    b3[4], b3[7], b3[2] = -5.0, -4.0, -3.0
     P[-K:] = b3
     L = pls.lib()
-    mean = np.zeros(12, np.float32)
+    rp, ci, v = gen.laplace2d(100)
+    A = pls.SpMat.csr(10000, 10000, rp, ci, v)
+    f = A.features()
+    mean = f.astype(np.float32)  # these features standardise to 0
     std = np.ones(12, np.float32)
     pls.check("set_model", L.perflibs_amd_set_selector_model(P.ctypes.data, mean.ctypes.data, std.ctypes.data))
     try:
-        rp, ci, v = gen.laplace2d(100)
-        A = pls.SpMat.csr(10000, 10000, rp, ci, v)
-        A.select_mode(pls.SELECT_LEARNED)
+        A.select_mode(pls.SELECT_NETWORK)
         A.optimize()
-        assert A.variant() == 4
+        assert A.variant() == 4 and A.select_info() == (pls.PATH_NETWORK, 0)
+        lo, hi = (f - 1).astype(np.float32), (f + 1).astype(np.float32)
+        pls.check("range", L.perflibs_amd_set_selector_range(lo.ctypes.data, hi.ctypes.data))
+        B = pls.SpMat.csr(10000, 10000, rp, ci, v)
+        B.select_mode(pls.SELECT_LEARNED)
+        B.optimize()
+        assert B.select_info() == (pls.PATH_AUDITION, 3) and B.variant() in (4, 7, 2)
+        # a matrix whose features lie outside the range: every candidate timed
+        rp2, ci2, v2 = gen.powerlaw(1 << 15, 5)
+        Cm = pls.SpMat.csr(1 << 15, 1 << 15, rp2, ci2, v2)
+        Cm.select_mode(pls.SELECT_LEARNED)
+        Cm.optimize()
+        path, ntimed = Cm.select_info()
+        assert path == pls.PATH_EXHAUSTIVE and ntimed >= 4
+        ms = Cm.time_variants(1, 3)
+        assert ms[Cm.variant()] > 0 and ntimed == int((ms > 0).sum())
     finally:
         L.perflibs_amd_clear_selector_model()
 
 
-def test_shipped_model_picks_like_numpy(pls, gen, tmp_path):
+def test_shipped_model_picks_like_numpy(pls, orc, gen, tmp_path):
     from pathlib import Path
 
     from plsparse import selector as S
@@ -156,14 +199,24 @@ def test_shipped_model_picks_like_numpy(pls, gen, tmp_path):
         mm, n, rp, ci, v, _ = S.sample_matrix(kind, 900 + seed, rows=(m, m + 1))
         A = pls.SpMat.csr(mm, n, rp, ci, v)
         A.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
+        A.select_mode(pls.SELECT_NETWORK)
         f = A.features()
-        A.optimize()  # default mode: learned, since a model is loaded
+        A.optimize()
         _, _, pred = mlp.forward(params, ((f - mean) / std)[None, :], K)
         cand = np.ones(K, bool)
+        cand[2:10] = sell_rule(orc, mm, n, rp, ci)
         cand[10] = f[9] >= 0.75 and mm % 4 == 0
         cand[11] = f[10] >= 0.75 and mm % 8 == 0
         want = int(np.argmin(np.where(cand, pred[0], np.inf)))
         assert A.variant() == want, (kind, pls.VARIANTS[A.variant()], pls.VARIANTS[want])
+        # the default mode (a model is loaded) is the learned audition of the best three
+        D = pls.SpMat.csr(mm, n, rp, ci, v)
+        D.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
+        D.optimize()
+        path, ntimed = D.select_info()
+        top3 = np.argsort(np.where(cand, pred[0], np.inf), kind="stable")[:3]
+        assert (path == pls.PATH_AUDITION and ntimed == min(3, int(cand.sum()))
+                and D.variant() in top3) or path == pls.PATH_EXHAUSTIVE
 
 
 def test_block_fill_features_match_numpy(pls):
