@@ -171,12 +171,13 @@ def test_learned_mode_uses_model(pls, gen):
         B.optimize()
         assert B.select_info() == (pls.PATH_AUDITION, 3) and B.variant() in (4, 7, 2)
         # a matrix whose features lie outside the range: every candidate timed
+        # (on power-law rows the padding rule leaves no SELL-C-sigma candidate)
         rp2, ci2, v2 = gen.powerlaw(1 << 15, 5)
         Cm = pls.SpMat.csr(1 << 15, 1 << 15, rp2, ci2, v2)
         Cm.select_mode(pls.SELECT_LEARNED)
         Cm.optimize()
         path, ntimed = Cm.select_info()
-        assert path == pls.PATH_EXHAUSTIVE and ntimed >= 4
+        assert path == pls.PATH_EXHAUSTIVE and ntimed >= 2
         ms = Cm.time_variants(1, 3)
         assert ms[Cm.variant()] > 0 and ntimed == int((ms > 0).sum())
     finally:

@@ -60,7 +60,9 @@ def test_candidate_rule_skips_padded_sell(pls, orc, gen):
     """Exhaustive selection times only SELL layouts that pass the rule; the
     same rule holds for the learned and heuristic modes (variant_candidate)."""
     rng = np.random.default_rng(5)
-    lens = np.where(np.arange(64 * 600) % 64 == 0, 200, 4)  # one long row per 64: sigma = 0 pads
+    # one long row per 64 in two whole sort windows of 16384 rows: sigma = 0 pads
+    # every chunk, sigma = 16384 gathers the long rows into unpadded chunks
+    lens = np.where(np.arange(2 * 16384) % 64 == 0, 200, 4)
     m = n = len(lens)
     rp, ci, v = lengths_csr(rng, lens, n)
     A = pls.SpMat.csr(m, n, rp, ci, v)
@@ -69,8 +71,7 @@ def test_candidate_rule_skips_padded_sell(pls, orc, gen):
         ref = orc.scs_beta_threads(m, n, rp, ci, Cc, sigma, 256)
         ok = ref["beta"] >= 0.96 and ref["min_beta"] > 0.85
         assert (ms[var] > 0) == ok, (Cc, sigma, ref["beta"], ref["min_beta"], ms[var])
-    # sigma = 0 pads every chunk: both C fail; sigma = 16384 sorts the long rows together
-    assert ms[2] < 0 and ms[6] < 0 and ms[5] > 0
+    assert ms[2] < 0 and ms[6] < 0 and ms[5] > 0 and ms[9] > 0
     # a stencil pads only at the boundary rows: every SELL variant stays a candidate
     rp, ci, v = gen.laplace2d(200)
     L = pls.SpMat.csr(200 * 200, 200 * 200, rp, ci, v)
