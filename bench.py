@@ -355,8 +355,8 @@ def spmm_bench(args) -> int:
 
 
 def mtx_bench(args) -> int:
-    """FP64 SpMV on a Matrix Market file (read like the reference's benchmark
-    driver, bench/pl_bench_sparse_x_dense.c:124-240): optimize() selects a
+    """FP64 SpMV on a Matrix Market file (read with the acceptance rules of the
+    reference's benchmark reader, bench/pl_bench_sparse_x_dense.c:124-240): optimize() selects a
     variant, every applicable variant is timed for the regret, x uniform[-1, 1]."""
     import numpy as np
     import torch

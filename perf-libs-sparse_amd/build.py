@@ -69,14 +69,17 @@ def build_workloads() -> Path:
 
 
 def build_apps() -> Path:
-    """The C benchmark driver (apps/sparse_x_dense.c), linked against both
-    in-tree libraries with an $ORIGIN rpath so it runs from lib/."""
+    """The C driver apps/mtx_roofline.c, linked against both in-tree libraries
+    (and the HIP runtime, for its device-resident vectors) with an $ORIGIN
+    rpath so it runs from lib/."""
     LIB.mkdir(exist_ok=True)
-    out = LIB / "sparse_x_dense"
-    tmp = LIB / "sparse_x_dense.tmp"
-    _run(["gcc", "-O2", "-std=c99", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'workloads'}",
-          str(PKG / "apps" / "sparse_x_dense.c"), f"-L{LIB}", "-lplsparse_amd",
-          "-lplsparse_workloads", "-lm", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    out = LIB / "mtx_roofline"
+    tmp = LIB / "mtx_roofline.tmp"
+    _run(["gcc", "-O2", "-std=c99", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}",
+          f"-I{PKG / 'workloads'}", "-I/opt/rocm/include",
+          str(PKG / "apps" / "mtx_roofline.c"), f"-L{LIB}", "-lplsparse_amd",
+          "-lplsparse_workloads", "-L/opt/rocm/lib", "-lamdhip64", "-lm",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

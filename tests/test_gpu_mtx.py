@@ -1,7 +1,8 @@
 """Real-matrix ingest on the GPU: a Matrix Market file read by workloads/mtx.cpp
-drives the SpMV path (checked against the oracle), and the C benchmark driver
-apps/sparse_x_dense.c (the reference's bench/pl_bench_sparse_x_dense.c flow:
-read, create A/B/C, spmm_optimize, 10 x spmm_exec) runs on the library."""
+drives the SpMV path (checked against the oracle), and the plain C caller
+apps/mtx_roofline.c (read, optimize, SpMV on device vectors beside the drop-in
+call, SpMM on dense handles, each against the HBM roofline) runs on the
+library."""
 from __future__ import annotations
 
 import subprocess
@@ -18,7 +19,7 @@ from plsparse import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-DRIVER = Path(__file__).resolve().parent.parent / "perf-libs-sparse_amd" / "lib" / "sparse_x_dense"
+DRIVER = Path(__file__).resolve().parent.parent / "perf-libs-sparse_amd" / "lib" / "mtx_roofline"
 
 
 def stencil_file(tmp_path, n=16, sym="general"):
@@ -47,20 +48,40 @@ def test_spmv_on_read_matrix(pls, orc, tmp_path, sym):
     assert rel_err(y, orc.spmv(m, rp, ci, v, x)) <= 1e-12
 
 
+def _fields(line):
+    head, *kv = line.split()
+    return head, dict(f.split("=", 1) for f in kv)
+
+
 def test_benchmark_driver(tmp_path):
     assert DRIVER.exists(), "build the driver with perf-libs-sparse_amd/build.py"
     p, m = stencil_file(tmp_path, n=12)
     print(f"running {DRIVER.name} on {p.name}", flush=True)
-    r = subprocess.run([str(DRIVER), str(p)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(DRIVER), str(p), "1", "8", "33"], capture_output=True, text=True,
+                       timeout=120)
     sys.stdout.write(r.stdout)
     sys.stderr.write(r.stderr)
     assert r.returncode == 0
-    lines = r.stdout.splitlines()
+    lines = [_fields(l) for l in r.stdout.splitlines()]
     nnz = len(W.stencil27(12)[1])
-    assert lines[0] == f"A: {m} x {m} nnz={nnz}"
-    assert lines[1] == f"B: {m} x 1024" and lines[2] == f"C: {m} x 1024"
-    assert lines[3].startswith("iters=10 total_s=")
-    assert r.stderr == "" or "error" not in r.stderr
+    assert lines[0] == ("matrix", {"m": str(m), "n": str(m), "nnz": str(nnz)})
+    head, f = lines[1]
+    assert head == "spmv" and f["variant"] in pls_variants()
+    assert float(f["dropin_vs_device_rel"]) <= 1e-15  # same kernel on the same inputs
+    head, f = lines[2]
+    want = 12 * nnz + 4 * (m + 1) + 16 * m
+    assert head == "spmv" and int(f["bytes"]) == want and float(f["ms"]) > 0
+    assert abs(float(f["frac"]) - want / (float(f["ms"]) * 1e-3) / 8e12) <= 1e-3
+    for (head, f), k in zip(lines[3:], (1, 8, 33)):
+        assert head == "spmm" and int(f["k"]) == k and float(f["ms"]) > 0
+        assert int(f["bytes"]) == 12 * nnz + 4 * (m + 1) + 16 * m * k
+    assert len(lines) == 6
+
+
+def pls_variants():
+    import plsparse
+
+    return plsparse.VARIANTS
 
 
 def test_benchmark_driver_reports_reader_errors(tmp_path):
@@ -69,3 +90,4 @@ def test_benchmark_driver_reports_reader_errors(tmp_path):
     r = subprocess.run([str(DRIVER), str(p)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "complex Matrix Market files not supported" in r.stderr
+    assert r.stdout == ""
