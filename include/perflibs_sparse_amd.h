@@ -196,6 +196,23 @@ perflibs_status_t perflibs_amd_mlp_backward(const float *params_dev,
                                             const float *T_dev,
                                             const float *M_dev,
                                             float *grad_dev, float *loss_dev);
+/* perflibs_amd_mlp_backward without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, training loops that forbid hipMalloc).  The
+ * per-sample gradient scratch lives in a static device array of the library,
+ * one per GPU, sized for n <= PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES (larger n:
+ * status 1, nothing launched).  Calls that overlap on one GPU share that
+ * array and race; calls on one stream are ordered and safe.  Results are
+ * bit-identical to perflibs_amd_mlp_backward. */
+#define PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES 4096
+perflibs_status_t perflibs_amd_mlp_backward_noalloc(const float *params_dev,
+                                                    const float *X_dev, int n,
+                                                    const float *h1_dev,
+                                                    const float *h2_dev,
+                                                    const float *out_dev,
+                                                    const float *T_dev,
+                                                    const float *M_dev,
+                                                    float *grad_dev, float *loss_dev);
 perflibs_status_t perflibs_amd_mlp_adam(float *params_dev, const float *grad_dev,
                                         float *m_dev, float *v_dev, int step,
                                         float lr, float beta1, float beta2,

@@ -358,6 +358,50 @@ hipError_t mlp_backward(const float *P, const float *X, int n, int I, int H, int
   return hipGetLastError();
 }
 
+// Scratch of the allocation-free backward pass (perflibs_amd_mlp_backward_noalloc):
+// d_out [n x K], d_h2 [n x H] and d_h1 [n x H] for up to
+// PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES samples, 2.3 MB per GPU for the whole
+// process.  Every element a call reads is written earlier by the same call, so
+// nothing needs clearing between calls.
+constexpr int64_t kMlpScratchFloats =
+    int64_t(PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES) *
+    (PERFLIBS_AMD_NUM_VARIANTS + 2 * PERFLIBS_AMD_MLP_HIDDEN);
+static __device__ __align__(256) float scratch[kMlpScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+float *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+hipError_t mlp_noalloc_scratch(float **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = static_cast<float *>(p);
+  }
+  *ptr = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+hipError_t mlp_backward_noalloc(const float *P, const float *X, int n, int I, int H, int K,
+                                const float *h1, const float *h2, const float *out,
+                                const float *T, const float *M, float *G, float *loss_cnt,
+                                hipStream_t s) {
+  if (n <= 0 || n > PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES || K != PERFLIBS_AMD_NUM_VARIANTS ||
+      H != PERFLIBS_AMD_MLP_HIDDEN)
+    return hipErrorInvalidValue;
+  float *base = nullptr;
+  hipError_t e = mlp_noalloc_scratch(&base);
+  if (e != hipSuccess) return e;
+  float *dout = base, *dh2 = dout + int64_t(n) * K, *dh1 = dh2 + int64_t(n) * H;
+  return mlp_backward(P, X, n, I, H, K, h1, h2, out, T, M, G, loss_cnt, dout, dh2, dh1, s);
+}
+
 hipError_t adam_step(float *P, const float *G, float *m, float *v, int64_t np, int step, float lr,
                      float b1, float b2, float eps, hipStream_t s) {
   const float c1 = 1.f - powf(b1, float(step)), c2 = 1.f - powf(b2, float(step));

@@ -440,6 +440,12 @@ hipError_t mlp_backward(const float *P, const float *X, int n, int I, int H, int
                         const float *h1, const float *h2, const float *out, const float *T,
                         const float *M, float *G, float *loss_cnt, float *dout, float *dh2,
                         float *dh1, hipStream_t s);
+// the backward pass with its scratch in the library's static device array
+// (n <= PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES); allocates nothing
+hipError_t mlp_backward_noalloc(const float *P, const float *X, int n, int I, int H, int K,
+                                const float *h1, const float *h2, const float *out,
+                                const float *T, const float *M, float *G, float *loss_cnt,
+                                hipStream_t s);
 hipError_t adam_step(float *P, const float *G, float *m, float *v, int64_t np, int step, float lr,
                      float b1, float b2, float eps, hipStream_t s);
 }  // namespace plk

@@ -520,6 +520,24 @@ perflibs_status_t perflibs_amd_mlp_backward(const float *P, const float *X, int 
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_mlp_backward_noalloc(const float *P, const float *X, int n,
+                                                    const float *h1, const float *h2,
+                                                    const float *out, const float *T,
+                                                    const float *M, float *G, float *loss) {
+  if (!P || !X || !h1 || !h2 || !out || !T || !M || !G || !loss || n <= 0)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (n > PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES) {
+    pls::set_last_error("perflibs_amd_mlp_backward_noalloc: n = " + std::to_string(n) +
+                        " exceeds the static scratch limit of " +
+                        std::to_string(PERFLIBS_AMD_MLP_NOALLOC_MAX_SAMPLES) + " samples");
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  }
+  const hipError_t e = plk::mlp_backward_noalloc(P, X, n, pls::kF, pls::kH, pls::kK, h1, h2, out,
+                                                 T, M, G, loss, pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "MLP backward (static scratch)");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t perflibs_amd_mlp_adam(float *P, const float *G, float *m, float *v, int step,
                                         float lr, float beta1, float beta2, float eps) {
   if (!P || !G || !m || !v || step <= 0) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;

@@ -116,6 +116,7 @@ _SIGS = {
     "perflibs_amd_select_info": (_I, [_P, _P, _P]),
     "perflibs_amd_mlp_forward": (_I, [_P, _P, C.c_int, _P, _P, _P]),
     "perflibs_amd_mlp_backward": (_I, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_mlp_backward_noalloc": (_I, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_mlp_adam": (_I, [_P, _P, _P, _P, C.c_int, _F, _F, _F, _F]),
     "perflibs_amd_sell_info": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_sell_export": (_I, [_P, _P, _P, _P, _P, _P, _P]),

@@ -255,3 +255,14 @@ def test_introspection_without_plans(pls):
     assert lib.perflibs_amd_spmm_info(A.h, *[C.byref(x) for x in v[:3]], C.byref(tot)) == 1
     assert lib.perflibs_amd_panel_info(A.h, *[C.byref(x) for x in v]) == 1
     assert lib.perflibs_amd_spmm_info(None, None, None, None, None) == 1
+
+
+def test_noalloc_backward_refuses_sizes_beyond_its_scratch(pls):
+    """The allocation-free MLP backward checks n against its static scratch
+    (4096 samples) before touching the device; pointers are never read."""
+    L = pls.lib()
+    p = 0x1000
+    for n in (4097, 1 << 20):
+        assert L.perflibs_amd_mlp_backward_noalloc(p, p, n, p, p, p, p, p, p, p) == 1
+        assert b"static scratch" in L.perflibs_amd_last_error()
+    assert L.perflibs_amd_mlp_backward_noalloc(p, p, 0, p, p, p, p, p, p, p) == 1
