@@ -14,8 +14,13 @@ library's optimize() selected.  `value` is algorithmic GB/s of the timed steps:
 
 Also reported: every SpMV variant's GB/s on the same matrix (the exhaustive
 sweep) and the selected variant's regret, the HIP-event roofline of the
-dominant kernel, and the reference's CPU path (this project's OpenMP port of
-it) timed on the host.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+dominant kernel with the random-gather ceilings of the same matrix, and the
+reference's CPU path (this project's OpenMP port of it) timed on the host.
+`extra_lines` carries the other configurations with their algorithmic bytes
+and HBM fractions: FP32 and BF16 value storage on the same matrix, the 5-point
+Laplacian 316^2 on the reference's CPU path (and, cache-resident, on the GPU),
+CSR/BSR x dense SpMM on the 27-point stencil 128^3, and the batch of 4096
+banded products.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -140,7 +145,7 @@ def cpu_baseline(n, rp, ci, v, x, y_gpu) -> dict:
     }
 
 
-def batch_bench(args) -> int:
+def batch_lines(nprob: int, steps: int, warmup: int, with_cpu: bool) -> dict:
     """The batched configuration: 4096 banded problems (rows log-uniform in
     [256, 65536], bandwidth 32-512, 5-40 non-zeros per row, k in {1, 4, 16}) as
     one persistent launch replayed from a hipGraph, for FP64, FP32 and BF16
@@ -155,7 +160,7 @@ def batch_bench(args) -> int:
 
     P.set_stream(torch.cuda.current_stream().cuda_stream)
     t0 = time.perf_counter()
-    probs = W.banded_batch(args.batch, 0x5EED0004)
+    probs = W.banded_batch(nprob, 0x5EED0004)
     nnz = sum(int(p[1][-1]) for p in probs)
     log(f"{len(probs)} problems, {sum(p[0] for p in probs)} rows, nnz={nnz} in {time.perf_counter() - t0:.1f}s")
     rng = np.random.default_rng(0x5EED1004)
@@ -173,16 +178,16 @@ def batch_bench(args) -> int:
         t0 = time.perf_counter()
         B.optimize(autotune=True)
         tune_s = time.perf_counter() - t0
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             B.exec(1.0, 0.0)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             B.exec(1.0, 0.0)
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
+        ms = e0.elapsed_time(e1) / steps
         sv = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
         sx = 8 if prec == "fp64" else 4
         nbytes = sum(int(rp[-1]) * (sv + 4) + (m + 1) * 4 + m * k * sx * 2 for (m, rp, _, _, k) in probs)
@@ -200,16 +205,26 @@ def batch_bench(args) -> int:
         B.destroy()
         del mats, Y
         torch.cuda.empty_cache()
+    for r in results.values():
+        r["hbm_fraction"] = round(r["GB/s"] / HBM_PEAK_GBS, 4)
+    cpu = batch_cpu_baseline(probs, X64, y64, results["fp64"]["algorithmic_bytes"]) if with_cpu else None
+    del X64, probs
+    torch.cuda.empty_cache()
+    return {"problems": nprob, "nnz": nnz, "rhs": [1, 4, 16], "seed": 0x5EED0004,
+            "precisions": results, "cpu_baseline": cpu}
+
+
+def batch_bench(args) -> int:
+    r = batch_lines(args.batch, args.steps, args.warmup, not args.no_cpu)
     out = {
-        "metric": METRIC, "value": results["fp64"]["GB/s"], "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["fp64"]["ms"],
+        "metric": METRIC, "value": r["precisions"]["fp64"]["GB/s"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["precisions"]["fp64"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "batched_banded_4096", "problems": len(probs), "nnz": nnz,
-                   "rhs": [1, 4, 16], "seed": 0x5EED0004},
-        "precisions": results,
-        "cpu_baseline": None if args.no_cpu else batch_cpu_baseline(
-            probs, X64, y64, results["fp64"]["algorithmic_bytes"]),
+        "config": {"workload": "batched_banded_4096", "problems": r["problems"], "nnz": r["nnz"],
+                   "rhs": r["rhs"], "seed": r["seed"]},
+        "precisions": r["precisions"],
+        "cpu_baseline": r["cpu_baseline"],
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -286,7 +301,7 @@ def spmm_cpu_baseline(rp, ci, v, k, nbytes) -> dict:
     }
 
 
-def spmm_bench(args) -> int:
+def spmm_lines(steps: int, warmup: int, with_cpu: bool) -> dict:
     """CSR x dense SpMM: the 27-point stencil on 128^3 (n = 2,097,152) times a
     row-major B of k in {8, 32, 128} columns, FP64 and FP32, and the BSR
     operator of the same shape (27-point block pattern of dense N(0,1) 4x4 or
@@ -308,17 +323,18 @@ def spmm_bench(args) -> int:
     def run(tag, A, k, tdt, nbytes):
         Bd = torch.rand((n, k), dtype=tdt, device="cuda") * 2 - 1
         Cd = torch.empty((n, k), dtype=tdt, device="cuda")
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k)
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
+        ms = e0.elapsed_time(e1) / steps
         lines[tag] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
+                      "hbm_fraction": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
                       "GFLOP/s": None, "algorithmic_bytes": nbytes}
         log(f"{tag}: {ms:.3f} ms {nbytes / ms / 1e6:.1f} GB/s")
         del Bd, Cd
@@ -340,16 +356,206 @@ def spmm_bench(args) -> int:
             lines[f"bsr{b}_fp64_k{k}"]["GFLOP/s"] = round(2 * nnzb * b * b * k / lines[f"bsr{b}_fp64_k{k}"]["ms"] / 1e6, 1)
         A.destroy()
         del brp, bci, bv
-    best = lines["csr_fp64_k32"]
-    cpu = None if args.no_cpu else spmm_cpu_baseline(rp, ci, v, 32, best["algorithmic_bytes"])
+    cpu = spmm_cpu_baseline(rp, ci, v, 32, lines["csr_fp64_k32"]["algorithmic_bytes"]) if with_cpu else None
+    return {"rows": n, "nnz": nnz, "lines": lines, "cpu_baseline": cpu}
+
+
+def spmm_bench(args) -> int:
+    r = spmm_lines(args.steps, args.warmup, not args.no_cpu)
+    best = r["lines"]["csr_fp64_k32"]
     out = {
         "metric": METRIC, "value": best["GB/s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "spmm_stencil27_128", "rows": n, "nnz": nnz, "value_line": "csr_fp64_k32"},
-        "lines": lines,
-        "cpu_baseline": cpu,
+        "config": {"workload": "spmm_stencil27_128", "rows": r["rows"], "nnz": r["nnz"],
+                   "value_line": "csr_fp64_k32"},
+        "lines": r["lines"],
+        "cpu_baseline": r["cpu_baseline"],
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def reduced_precision_lines(rp, ci, v, x, y64, steps: int, warmup: int) -> dict:
+    """FP32 and BF16 value storage on the headline matrix (FP32 x and y, FP32
+    accumulation; optimize() selects the variant): time, algorithmic GB/s with
+    s_v = 4 or 2 and s_x = s_y = 4, and the normwise error against the FP64
+    result of the same matrix and x."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    out = {}
+    v32 = v.astype(np.float32)
+    xd = torch.from_numpy(x.astype(np.float32)).cuda()
+    yd = torch.zeros(n, dtype=torch.float32, device="cuda")
+    for tag, prec, sv in (("fp32", P.VALUES_NATIVE, 4), ("bf16", P.VALUES_BF16, 2)):
+        A = P.SpMat.csr(n, n, rp, ci, v32, flags=P.CREATE_NOCOPY)
+        A.hint(P.HINT_SPMV_INVOCATIONS, P.INVOCATIONS_MANY)
+        A.set_precision(prec)
+        A.optimize()
+        _, kms = time_gpu(A, xd, yd, warmup, steps)
+        y = yd.double().cpu().numpy()
+        nbytes = nnz * (sv + 4) + (n + 1) * 4 + n * 4 * 2
+        out[tag] = {"variant": P.VARIANTS[A.variant()], "ms": round(kms, 4),
+                    "GB/s": round(nbytes / kms / 1e6, 1),
+                    "hbm_fraction": round(nbytes / kms / 1e6 / HBM_PEAK_GBS, 4),
+                    "algorithmic_bytes": nbytes,
+                    "rel_err_vs_fp64": float(np.linalg.norm(y - y64) / np.linalg.norm(y64))}
+        log(f"spmv {tag}: {out[tag]}")
+        A.destroy()
+    return out
+
+
+def laplace_line(steps: int) -> dict:
+    """The 5-point Laplacian on 316 x 316 (n = 99,856, 498,016 non-zeros,
+    y = A x + 0.5 y, FP64) on the reference's CPU path (this project's OpenMP
+    port of its first-touch and vanilla CSR kernels, the faster reported),
+    and on the GPU (optimize() selects; cache-resident: the whole problem is
+    8.8 MB, far inside the L2 and MALL, so its GB/s is not an HBM figure)."""
+    import numpy as np
+    import torch
+
+    import plsparse as P
+    from oracle import oracle as O
+    from plsparse import workloads as W
+
+    rp, ci, v = W.laplace2d(316)
+    n = 316 * 316
+    nnz = int(rp[-1])
+    x = W.uniform(n, 0x5EED0001)
+    y0 = W.uniform(n, 0x5EED1001)
+    nbytes = nnz * 12 + (n + 1) * 4 + n * 8 + n * 8 * 2
+    L = O.lib()
+    threads = O.omp_threads()
+    p = lambda a: a.ctypes.data
+    rows, inds = O.csr_decomp(n, rp, threads, 1)
+    y = y0.copy()
+    cpu = {}
+    for name, fn in (("csr_first_touch", lambda: L.orc_cpu_csr_first_touch(
+            n, p(rp), p(ci), p(v), p(x), 1.0, 0.5, p(y), threads, p(rows), p(inds))),
+            ("csr_vanilla", lambda: L.orc_cpu_csr_vanilla(n, p(rp), p(ci), p(v), p(x), 1.0, 0.5, p(y)))):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        cpu[name] = float(np.median(ts))
+    best = min(cpu, key=cpu.get)
+    ref = O.spmv(n, rp, ci, v, x, 1.0, 0.5, y0)
+    A = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+    A.hint(P.HINT_SPMV_INVOCATIONS, P.INVOCATIONS_MANY)
+    A.optimize()
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.from_numpy(y0.copy()).cuda()
+    A.spmv_async(xd.data_ptr(), yd.data_ptr(), 1.0, 0.5)
+    torch.cuda.synchronize()
+    err = float(np.linalg.norm(yd.cpu().numpy() - ref) / np.linalg.norm(ref))
+    for _ in range(5):
+        A.spmv_async(xd.data_ptr(), yd.data_ptr(), 1.0, 0.5)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        A.spmv_async(xd.data_ptr(), yd.data_ptr(), 1.0, 0.5)
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / steps
+    out = {"rows": n, "nnz": nnz, "alpha": 1.0, "beta": 0.5, "algorithmic_bytes": nbytes,
+           "cpu_path": {"kernel": best, "ms": round(cpu[best] * 1e3, 4),
+                        "GB/s": round(nbytes / cpu[best] / 1e9, 2), "cores": threads,
+                        "all_ms": {k: round(t * 1e3, 4) for k, t in cpu.items()}},
+           "gpu_cache_resident": {"variant": P.VARIANTS[A.variant()], "ms": round(gms, 4),
+                                  "GB/s": round(nbytes / gms / 1e6, 1), "rel_err_vs_oracle": err}}
+    A.destroy()
+    log(f"laplace 316^2: {out}")
+    return out
+
+
+def gather_ceiling(A_panel, xd, nnz: int, kern_ms: float, steps: int) -> dict:
+    """Ceilings of the headline SpMV set by its x gathers: (1) the panel
+    kernel's own index/value stream and x gathers without the row sums
+    (perflibs_amd_probe_panel_gathers, same plan and grid); (2) nnz random
+    8-byte gathers from a 512 KiB table (L2-resident) and (3) from a 32 MiB
+    table (the size of x, MALL-resident), with no index stream
+    (perflibs_amd_probe_random_gathers).  Each is HIP-event timed on the
+    library stream over `steps` launches."""
+    import torch
+
+    import plsparse as P
+
+    L = P.lib()
+    info = A_panel.panel_info()
+    out_p = torch.empty(max(info["grid"], 1) * 256, dtype=torch.float64, device="cuda")
+    out_r = torch.empty(8192 * 256, dtype=torch.float64, device="cuda")
+    tab_l2 = torch.rand(1 << 16, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    t_panel = timed(lambda: P.check("probe_panel_gathers", L.perflibs_amd_probe_panel_gathers(
+        A_panel.h, xd.data_ptr(), out_p.data_ptr())))
+    t_l2 = timed(lambda: P.check("probe_random_gathers", L.perflibs_amd_probe_random_gathers(
+        tab_l2.data_ptr(), 1 << 16, nnz, out_r.data_ptr())))
+    mall_len = 1 << 22
+    tab_mall = xd if xd.numel() >= mall_len else torch.rand(mall_len, dtype=torch.float64, device="cuda")
+    t_mall = timed(lambda: P.check("probe_random_gathers", L.perflibs_amd_probe_random_gathers(
+        tab_mall.data_ptr(), mall_len, nnz, out_r.data_ptr())))
+    out = {"gathers": nnz,
+           "panel_streams_and_gathers_ms": round(t_panel, 4),
+           "random_gathers_l2_ms": round(t_l2, 4), "random_gathers_l2_per_s": round(nnz / t_l2 / 1e6, 1),
+           "random_gathers_mall_ms": round(t_mall, 4),
+           "random_gathers_mall_per_s": round(nnz / t_mall / 1e6, 1),
+           "kernel_ms": round(kern_ms, 4),
+           "frac_of_l2_gather_ceiling": round(t_l2 / kern_ms, 4),
+           "frac_of_panel_stream_gather_ceiling": round(t_panel / kern_ms, 4)}
+    log(f"gather ceiling: {out}")
+    return out
+
+
+def gather_bench(args) -> int:
+    """The gather workload: the headline matrix's column-panel plan, its SpMV
+    and the gather probes of gather_ceiling() (G gathers per second), one JSON
+    line whose value is the L2 random-gather rate in G gathers/s."""
+    import torch
+
+    import plsparse as P
+    from plsparse import workloads as W
+
+    n = args.rows
+    rp, ci, v = W.powerlaw(n, SEED_A)
+    x = W.uniform(n, SEED_X)
+    nnz = int(rp[-1])
+    P.set_stream(torch.cuda.current_stream().cuda_stream)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    A = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+    A.force_variant(P.VARIANTS.index("csr_panel"))
+    A.optimize()
+    _, kms = time_gpu(A, xd, yd, args.warmup, args.steps)
+    g = gather_ceiling(A, xd, nnz, kms, args.steps)
+    A.destroy()
+    out = {"metric": "random 8-byte gathers per second (L2-resident table)",
+           "value": round(nnz / g["random_gathers_l2_ms"] / 1e6, 1), "unit": "Ggather/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": g["random_gathers_l2_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "gather_powerlaw_4m", "rows": n, "nnz": nnz, "seed": SEED_A},
+           "gather_ceiling": g}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -418,9 +624,12 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-variant exhaustive sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--rows", type=int, default=N_ROWS)
-    ap.add_argument("--workload", choices=("spmv", "batch", "spmm"), default="spmv",
+    ap.add_argument("--workload", choices=("spmv", "batch", "spmm", "gather"), default="spmv",
                     help="spmv: the headline SpMV line (default); batch: the batched configuration; "
-                         "spmm: CSR/BSR x dense on the 27-point stencil")
+                         "spmm: CSR/BSR x dense on the 27-point stencil; gather: the random-gather "
+                         "ceilings of the headline matrix")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline line only: skip the SpMM, batch, FP32/BF16, Laplacian and gather lines")
     ap.add_argument("--batch", type=int, default=4096, help="problems of the batched workload")
     ap.add_argument("--mtx", default=None, help="FP64 SpMV on this Matrix Market file instead")
     args = ap.parse_args()
@@ -432,6 +641,8 @@ def main() -> int:
         return batch_bench(args)
     if args.workload == "spmm":
         return spmm_bench(args)
+    if args.workload == "gather":
+        return gather_bench(args)
     if args.mtx:
         return mtx_bench(args)
 
@@ -468,6 +679,17 @@ def main() -> int:
     value = nbytes / (wall_ms * 1e-3) / 1e9
     achieved = nbytes / (kern_ms * 1e-3) / 1e9
     log(f"{chosen}: {wall_ms:.3f} ms/step wall, {kern_ms:.3f} ms kernel, {achieved:.1f} GB/s")
+    gather = None
+    if not args.no_extras:
+        if chosen == "csr_panel":
+            gather = gather_ceiling(A, xd, nnz, kern_ms, 20)
+        else:
+            G = P.SpMat.csr(n, n, rp, ci, v, flags=P.CREATE_NOCOPY)
+            G.force_variant(P.VARIANTS.index("csr_panel"))
+            G.optimize()
+            _, gms = time_gpu(G, xd, yd, 3, 20)
+            gather = gather_ceiling(G, xd, nnz, gms, 20)
+            G.destroy()
     A.destroy()
 
     sweep = {}
@@ -488,6 +710,21 @@ def main() -> int:
     cpu = None if args.no_cpu else cpu_baseline(n, rp, ci, v, x, y_gpu)
     workload = "spmv_fp64_powerlaw_4m" if n == N_ROWS else f"spmv_fp64_powerlaw_{n}"
     traffic = measured_traffic(chosen, workload)
+
+    # the other configurations, each with its algorithmic bytes and HBM fraction
+    # (fewer repetitions than the headline, so the whole run stays in minutes)
+    extras = None
+    if not args.no_extras:
+        extras = {"spmv_powerlaw_4m_reduced_precision": reduced_precision_lines(rp, ci, v, x, y_gpu, 20, 3)}
+        del xd, yd
+        torch.cuda.empty_cache()
+        extras["laplace2d_316"] = laplace_line(50)
+        t0 = time.perf_counter()
+        extras["spmm_stencil27_128"] = spmm_lines(10, 2, False)
+        log(f"spmm lines in {time.perf_counter() - t0:.1f}s")
+        t0 = time.perf_counter()
+        extras["batched_banded_4096"] = batch_lines(args.batch, 10, 2, False)
+        log(f"batch lines in {time.perf_counter() - t0:.1f}s")
 
     out = {
         "metric": METRIC,
@@ -515,11 +752,13 @@ def main() -> int:
             "traffic": traffic[0] if traffic else None,
             "traffic_source": f"profiles/{traffic[1]}" if traffic else None,
             "kernel": chosen, "kernel_ms": round(kern_ms, 4), "algorithmic_bytes": nbytes,
+            "gather_ceiling": gather,
         },
         "cpu_baseline": cpu,
         "variants_gbs": {k: round(nbytes / ms / 1e6, 1) for k, ms in sweep.items()},
         "selector": {"picked": chosen, "exhaustive_best": best,
                      "regret": None if regret is None else round(regret, 4)},
+        "extra_lines": extras,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -126,6 +126,26 @@ perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *pa
                                           perflibs_int_t *rows_per_tile_max,
                                           perflibs_int_t *grid);
 
+/* Measurement, not a product path: runs the column-panel kernel's index/value
+ * stream and x gathers over the handle's panel plan (built by an optimize
+ * with PERFLIBS_AMD_CSR_PANEL forced) without the row sums, on the library
+ * stream and not synchronized.  out_dev receives one partial sum per thread:
+ * grid * 256 values of the handle's type (grid from perflibs_amd_panel_info).
+ * Its time is the ceiling the gathers alone set on the panel SpMV.  Status 1
+ * without a panel plan. */
+perflibs_status_t perflibs_amd_probe_panel_gathers(perflibs_spmat_t A, const void *x_dev,
+                                                  void *out_dev);
+
+/* Measurement, not a product path: `count` random 8-byte gathers from
+ * table_dev[table_len] (a power of two; indices from an integer hash, 8 in
+ * flight per thread), on the library stream, not synchronized.  out_dev
+ * receives PERFLIBS_AMD_PROBE_GRID * 256 partial sums.  A table that fits an
+ * XCD's L2 (4 MB) gives the random-gather rate of the L1 -> L2 path, a larger
+ * one that of the MALL. */
+#define PERFLIBS_AMD_PROBE_GRID 8192
+perflibs_status_t perflibs_amd_probe_random_gathers(const double *table_dev, int64_t table_len,
+                                                   int64_t count, double *out_dev);
+
 /* Matrix features used by the selector (PERFLIBS_AMD_NUM_FEATURES floats),
  * computed on the GPU from the CSR structure. */
 perflibs_status_t perflibs_amd_features(perflibs_spmat_t A, float *features);

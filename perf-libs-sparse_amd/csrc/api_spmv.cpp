@@ -299,6 +299,26 @@ perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blo
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_probe_panel_gathers(perflibs_spmat_t A, const void *x_dev,
+                                                  void *out_dev) {
+  if (!A || !x_dev || !out_dev || !A->plan.panel) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  const hipError_t e =
+      plk::panel_gathers(*A->plan.panel, A->dt == pls::DType::F64, x_dev, out_dev, pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(A, e, "panel gather probe");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_probe_random_gathers(const double *table_dev, int64_t table_len,
+                                                   int64_t count, double *out_dev) {
+  if (!table_dev || !out_dev || count < 0 || table_len <= 0 || table_len > (int64_t(1) << 32) ||
+      (table_len & (table_len - 1)) != 0)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  const hipError_t e = plk::random_gathers(table_dev, table_len, count, out_dev,
+                                           PERFLIBS_AMD_PROBE_GRID, pls::stream());
+  if (e != hipSuccess) return pls::hip_fail(nullptr, e, "random gather probe");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,
                                           perflibs_int_t *rows_per_tile_max,

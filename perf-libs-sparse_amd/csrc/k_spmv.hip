@@ -413,6 +413,75 @@ __global__ __launch_bounds__(kBlock) void k_csr_panel(
   }
 }
 
+// Measurement kernel, not used by any product path: the column-panel
+// kernel's loop with its index/value stream, its segment-table reads and its
+// x gathers unchanged (same layout, grid and LDS reservation, so the same
+// occupancy), the products summed per lane instead of per row.  Its time is
+// the part of the panel SpMV set by the streams and gathers alone.
+template <class V, class X>
+__global__ __launch_bounds__(kBlock) void k_panel_gathers(
+    int32_t ngroups, int32_t npanels, int32_t pw, const int64_t *__restrict__ seg,
+    const uint32_t *__restrict__ idx, const V *__restrict__ val, const X *__restrict__ x,
+    acc_t<X> *__restrict__ out) {
+  using A = acc_t<X>;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t cmask = (1u << pw) - 1u;
+  A sum = A(0);
+  for (int32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int32_t t = g * kPanelWaves + wave;
+    const int64_t *sg = seg + int64_t(t) * npanels;
+    const int64_t te = sg[npanels] & ~int64_t(3);
+    int32_t p = 0;
+    for (int64_t b = sg[0] & ~int64_t(3); b < te; b += 256) {
+      while ((sg[p + 1] & ~int64_t(3)) <= b) ++p;
+      int32_t pe = p;
+      while (pe + 1 < npanels && (sg[pe + 1] & ~int64_t(3)) <= min(b + 252, te - 4)) ++pe;
+      const int64_t q = b + 4 * lane;
+      const int64_t qc = min(q, te - 4);
+      const u32x4_t id = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(idx + qc));
+      A a[4];
+      ld_quad(val, qc, a);
+      int32_t pl = p;
+      for (int32_t k = p + 1; k <= pe; ++k) pl += qc >= (sg[k] & ~int64_t(3)) ? 1 : 0;
+      const X *xp = x + (int64_t(pl) << pw);
+      const uint32_t ids[4] = {id.x, id.y, id.z, id.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sum += q < te ? a[i] * A(xp[ids[i] & cmask]) : A(0);
+    }
+  }
+  out[int64_t(blockIdx.x) * kBlock + threadIdx.x] = sum;
+}
+
+// Measurement kernel: `count` random 8-byte gathers from a table of
+// mask + 1 doubles (indices from an integer hash of the gather number, so no
+// index stream), 8 independent gathers in flight per thread, one partial sum
+// per thread.  With a table that fits an XCD's L2 it measures the random
+// gather rate of the L1 -> L2 path; with a larger one, that of the MALL.
+__device__ __forceinline__ uint32_t gather_hash(uint32_t a) {
+  a ^= a >> 16;
+  a *= 0x7feb352du;
+  a ^= a >> 15;
+  a *= 0x846ca68bu;
+  a ^= a >> 16;
+  return a;
+}
+__global__ __launch_bounds__(kBlock) void k_random_gathers(int64_t count, uint32_t mask,
+                                                           const double *__restrict__ tab,
+                                                           double *__restrict__ out) {
+  constexpr int U = 8;
+  const int64_t tid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t nt = int64_t(gridDim.x) * kBlock;
+  double acc = 0.0;
+  for (int64_t i = tid; i < count / U; i += nt) {
+    double v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = tab[gather_hash(uint32_t(i * U + k)) & mask];
+#pragma unroll
+    for (int k = 0; k < U; ++k) acc += v[k];
+  }
+  out[tid] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // Vector epilogues of the exec shortcuts (src/matvec.cpp:76-85).
 template <class X>
@@ -636,6 +705,27 @@ hipError_t csr_panel(const pls::DevPanel &A, bool dbl, double alpha, const void 
           acc_t<X>(0));
     return hipGetLastError();
   });
+}
+
+hipError_t panel_gathers(const pls::DevPanel &A, bool dbl, const void *x, void *out,
+                         hipStream_t s) {
+  if (A.ntiles == 0 || A.grid == 0) return hipSuccess;
+  return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    const size_t lds = size_t(kPanelWaves) * A.rcap * sizeof(acc_t<X>);
+    k_panel_gathers<V, X><<<A.grid, kBlock, lds, s>>>(
+        A.ntiles / kPanelWaves, A.npanels, A.pw_log2, A.seg.as<int64_t>(), A.idx.as<uint32_t>(),
+        A.vals.as<V>(), (const X *)x, (acc_t<X> *)out);
+    return hipGetLastError();
+  });
+}
+
+hipError_t random_gathers(const double *table, int64_t table_len, int64_t count, double *out,
+                          int32_t grid, hipStream_t s) {
+  if (grid <= 0 || count <= 0) return hipSuccess;
+  k_random_gathers<<<grid, kBlock, 0, s>>>(count, uint32_t(table_len - 1), table, out);
+  return hipGetLastError();
 }
 
 int csr_panel_occupancy(bool dbl, pls::VStore vs, size_t lds_bytes) {

@@ -400,6 +400,14 @@ hipError_t csr_panel(const pls::DevPanel &A, bool dbl, double alpha, const void 
                      double beta, void *y, hipStream_t s);
 // Resident workgroups per CU of the column-panel kernel with `lds_bytes` of
 // accumulator LDS per workgroup (0 on error).
+// measurement only: the panel kernel's streams and x gathers without the row
+// sums; out holds one partial sum per thread of the grid (grid * 256 values)
+hipError_t panel_gathers(const pls::DevPanel &A, bool dbl, const void *x, void *out,
+                         hipStream_t s);
+// measurement only: count random gathers from table[table_len] (a power of
+// two), grid * 256 partial sums into out
+hipError_t random_gathers(const double *table, int64_t table_len, int64_t count, double *out,
+                          int32_t grid, hipStream_t s);
 int csr_panel_occupancy(bool dbl, pls::VStore vs, size_t lds_bytes);
 // y = beta*y (beta != 0) or y = 0 (beta == 0), n elements.
 hipError_t scale_vec(bool dbl, int64_t n, double beta, void *y, hipStream_t s);

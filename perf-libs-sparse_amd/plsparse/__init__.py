@@ -127,6 +127,8 @@ _SIGS = {
     "perflibs_amd_batch_set_shape": (_I, [_P, _I, _P, _P]),
     "perflibs_amd_batch_exec": (_I, [_P, _D, _D]),
     "perflibs_amd_batch_info": (_I, [_P, _P, _P, _P, _P, _P]),
+    "perflibs_amd_probe_panel_gathers": (_I, [_P, _P, _P]),
+    "perflibs_amd_probe_random_gathers": (_I, [_P, C.c_int64, C.c_int64, _P]),
     "perflibs_amd_last_error": (C.c_char_p, []),
 }
 
