@@ -222,6 +222,36 @@ perflibs_status_t spmm(int transA, int transB, T alpha, perflibs_spmat_top_t *A,
     if (tb) std::swap(b_sr, b_sc);
     const int64_t c_sr = cd.layout == PERFLIBS_ROW_MAJOR ? cd.lda : 1;
     const int64_t c_sc = cd.layout == PERFLIBS_COL_MAJOR ? cd.lda : 1;
+    // The SpMV of every column runs the plan that an SpMV with this op would
+    // run (the op-mismatch conversion first, as in spmv_exec).  A CSR-family
+    // plan runs all n columns in one launch that reads A once; other layouts
+    // (SELL-C-sigma, BSR, column panels) keep one SpMV per column.
+    if (transA != A->spmv_op || (A->fmt == Fmt::Scs && A->scs_op != transA)) {
+      pls::convert_to_csr(A);
+      A->spmv_op = transA;
+    }
+    st = pls::ensure_plan(A, transA);
+    if (st != PERFLIBS_STATUS_SUCCESS) return st;
+    if (A->plan.csr && A->plan.op == transA) {
+      const int p = plk::few_rhs_pitch(n);
+      const void *bp = bd.dev.get();
+      if (!((n == 1 || b_sc == 1) && b_sr == p)) {  // pack op(B) into rows of p values
+        const size_t bytes = esize(A) * size_t(k) * size_t(p);
+        if (A->stage_b.bytes() < bytes) e = A->stage_b.alloc(bytes);
+        for (int32_t j = 0; j < n && e == hipSuccess; ++j)
+          e = plk::strided_copy(dbl, k, static_cast<const char *>(bd.dev.get()) + esize(A) * size_t(j * b_sc),
+                                b_sr, static_cast<char *>(A->stage_b.get()) + esize(A) * size_t(j), p, s);
+        if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM column pack");
+        bp = A->stage_b.get();
+      }
+      e = plk::csr_spmm_few(*A->plan.csr, dbl, n, double(alpha), bp, double(beta), cd.dev.get(),
+                            c_sr, c_sc, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM");
+      cd.host_valid = false;
+      cd.matrix_is_zero = false;
+      return PERFLIBS_STATUS_SUCCESS;
+    }
     pls::DevBuf xcol, ycol;
     e = xcol.alloc(esize(A) * size_t(k));
     if (e == hipSuccess) e = ycol.alloc(esize(A) * size_t(m));

@@ -327,6 +327,72 @@ __global__ __launch_bounds__(kBlock) void k_spmm_strided(
   *c = X(r);
 }
 
+// Few right-hand sides (n < 5, the range where the reference repeats SpMV per
+// column, src/matmul.cpp:42-89): the CSR-vector mapping (L lanes per row, L
+// from the mean row length) with NR column sums per lane, so A is read once
+// for all columns.  op(B) arrives as a row-major block with rows of P >= NR
+// values (P in {1, 2, 4}, packed by the caller when its own layout is
+// strided), so each non-zero gathers one contiguous piece of B; C is written
+// through its strides (for the column-major C the reference materialises,
+// lanes of consecutive rows store consecutive addresses of each column).
+template <int P, class X> struct Piece;
+template <class X> struct Piece<1, X> {
+  static __device__ __forceinline__ void load(const X *b, acc_t<X> (&v)[1]) { v[0] = b[0]; }
+};
+template <class X> struct Piece<2, X> {
+  typedef X v2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ void load(const X *b, acc_t<X> (&v)[2]) {
+    const v2 t = *reinterpret_cast<const v2 *>(b);
+    v[0] = t.x;
+    v[1] = t.y;
+  }
+};
+template <class X> struct Piece<4, X> {
+  typedef X v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ void load(const X *b, acc_t<X> (&v)[4]) {
+    const v4 t = *reinterpret_cast<const v4 *>(b);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+  }
+};
+
+template <int L, int NR, int P, class V, class X, bool BETA>
+__global__ __launch_bounds__(kBlock) void k_csr_few_rhs(
+    int32_t m, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const V *__restrict__ val, const X *__restrict__ B, X *__restrict__ Cm, int64_t c_sr,
+    int64_t c_sc, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  const int64_t t = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  const int64_t row = t / L;
+  const int lane = threadIdx.x & (L - 1);
+  A s[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = A(0);
+  if (row < m) {
+    const int32_t e = rp[row + 1];
+    for (int32_t j = rp[row] + lane; j < e; j += L) {
+      const A a = A(ldv(val, j));
+      A b[P];
+      Piece<P, X>::load(B + int64_t(ldi(ci, j)) * P, b);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) s[r] += a * b[r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = group_sum_c<L>(s[r]);
+  if (row < m && lane == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      X *c = Cm + row * c_sr + r * c_sc;
+      A v = alpha * s[r];
+      if (BETA) v += beta * A(*c);
+      *c = X(v);
+    }
+  }
+}
+
 template <class X>
 __global__ __launch_bounds__(kBlock) void k_scale_dense(int64_t m, int64_t n, acc_t<X> beta,
                                                         X *Cm, int64_t sr, int64_t sc) {
@@ -454,6 +520,62 @@ hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha, con
         k_spmm_strided<V, X, false><<<grid, kBlock, 0, s>>>(
             A.m, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)B, b_sr,
             b_sc, (X *)C, c_sr, c_sc, acc_t<X>(alpha), acc_t<X>(0));
+    }
+    return hipGetLastError();
+  };
+  if (dbl) {
+    if (A.vs != pls::VStore::F64) return hipErrorInvalidValue;
+    return run((const double *)nullptr, (double *)nullptr);
+  }
+  if (A.vs == pls::VStore::F32) return run((const float *)nullptr, (float *)nullptr);
+  if (A.vs == pls::VStore::BF16) return run((const uint16_t *)nullptr, (float *)nullptr);
+  return hipErrorInvalidValue;
+}
+
+namespace {
+template <int L, int NR, int P, class V, class X>
+void launch_few(const pls::DevCsr &A, double alpha, const void *B, double beta, void *C,
+                int64_t c_sr, int64_t c_sc, hipStream_t s) {
+  const unsigned grid = grid_for(int64_t(A.m) * L);
+  if (grid == 0) return;
+  if (beta != 0.0)
+    k_csr_few_rhs<L, NR, P, V, X, true><<<grid, kBlock, 0, s>>>(
+        A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)B, (X *)C, c_sr,
+        c_sc, acc_t<X>(alpha), acc_t<X>(beta));
+  else
+    k_csr_few_rhs<L, NR, P, V, X, false><<<grid, kBlock, 0, s>>>(
+        A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(), (const X *)B, (X *)C, c_sr,
+        c_sc, acc_t<X>(alpha), acc_t<X>(0));
+}
+template <int NR, int P, class V, class X>
+void few_lanes(const pls::DevCsr &A, double alpha, const void *B, double beta, void *C,
+               int64_t c_sr, int64_t c_sc, hipStream_t s) {
+  switch (A.vec_lanes) {
+    case 1: launch_few<1, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    case 2: launch_few<2, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    case 4: launch_few<4, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    case 8: launch_few<8, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    case 16: launch_few<16, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    case 32: launch_few<32, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+    default: launch_few<64, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+  }
+}
+}  // namespace
+
+int few_rhs_pitch(int32_t n) { return n <= 1 ? 1 : n == 2 ? 2 : 4; }
+
+hipError_t csr_spmm_few(const pls::DevCsr &A, bool dbl, int32_t n, double alpha, const void *B,
+                        double beta, void *C, int64_t c_sr, int64_t c_sc, hipStream_t s) {
+  if (A.m == 0 || n <= 0) return hipSuccess;
+  if (n > 4) return hipErrorInvalidValue;
+  auto run = [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    switch (n) {
+      case 1: few_lanes<1, 1, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+      case 2: few_lanes<2, 2, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+      case 3: few_lanes<3, 4, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
+      default: few_lanes<4, 4, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
     }
     return hipGetLastError();
   };

@@ -422,3 +422,60 @@ def test_spmm_leading_dimensions(pls, orc, gen, k):
     assert rel_err(got[:, :k], ref) <= TOL64
     assert np.array_equal(got[:, k:], C0[:, k:])
     assert A.spmm_info()["blocks_state"] == 1
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("layout", ["row", "col"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmm_few_columns(pls, orc, gen, k, layout, dtype):
+    """k < 5 columns (the reference's repeated-SpMV range, src/matmul.cpp:42-89)
+    in one launch that reads A once: B and C row- or column-major with leading
+    dimensions wider than the matrix, beta 0 and 0.7, TRANS A, a null C
+    (materialised column-major), and a SELL-C-sigma A (one SpMV per column)."""
+    rng = np.random.default_rng(40 + k)
+    m = 3000
+    rp, ci, v = gen.powerlaw(m, 17)
+    v = v.astype(dtype)
+    vd = v.astype(np.float64)
+    tol = TOL64 if dtype == np.float64 else 1e-6
+    Bv = rng.uniform(-1, 1, (m, k)).astype(dtype).astype(np.float64)
+    C0 = rng.uniform(-1, 1, (m, k)).astype(dtype).astype(np.float64)
+
+    def dense(M):
+        if layout == "row":
+            ld = k + 3
+            buf = np.zeros((m, ld), dtype)
+            buf[:, :k] = M
+            return pls.SpMat.dense(pls.ROW_MAJOR, m, k, ld, buf.reshape(-1))
+        ld = m + 5
+        buf = np.zeros((k, ld), dtype)
+        buf[:, :m] = M.T
+        return pls.SpMat.dense(pls.COL_MAJOR, m, k, ld, buf.reshape(-1))
+
+    A = pls.SpMat.csr(m, m, rp, ci, v)
+    B = dense(Bv)
+    for beta in (0.0, 0.7):
+        Cm = dense(C0)
+        pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.25, A, B, beta, Cm)
+        ref = orc.spmm(m, k, rp, ci, vd, Bv, 1.25, beta, C0)
+        assert rel_err(Cm.export_dense(), ref) <= tol, beta
+    Cn = pls.SpMat.null(m, k, dtype)
+    pls.spmm_optimize(pls.NOTRANS, pls.NOTRANS, pls.SCALAR_ANY, A, B, pls.SCALAR_ANY, Cn)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, -2.0, A, B, 0.0, Cn)
+    assert rel_err(Cn.export_dense(), orc.spmm(m, k, rp, ci, vd, Bv, -2.0)) <= tol
+    # op(A) = A^T
+    D = np.zeros((m, m))
+    for i in range(m):
+        D[i, ci[rp[i]:rp[i + 1]]] += vd[rp[i]:rp[i + 1]]
+    At = pls.SpMat.csr(m, m, rp, ci, v)
+    Ct = dense(C0)
+    pls.spmm_exec(pls.TRANS, pls.NOTRANS, 0.5, At, B, 0.7, Ct)
+    assert rel_err(Ct.export_dense(), 0.5 * D.T @ Bv + 0.7 * C0) <= tol
+    # a SELL-C-sigma plan keeps one SpMV per column
+    As = pls.SpMat.csr(m, m, rp, ci, v)
+    As.force_sell(32, 1024)
+    As.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
+    As.optimize()
+    Cs = dense(C0)
+    pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.25, As, B, 0.7, Cs)
+    assert rel_err(Cs.export_dense(), orc.spmm(m, k, rp, ci, vd, Bv, 1.25, 0.7, C0)) <= tol
