@@ -550,7 +550,7 @@ void refresh_plan(perflibs_spmat_top_t *A, const std::vector<pls::ValueHit> &hit
   if (pls::patch_plan(A, hits)) return;
   const int keep = A->plan.variant;
   A->plan.clear();
-  if (keep >= 0 && A->forced_variant < 0 && A->forced_C < 0) A->forced_variant = keep;
+  if (keep >= 0 && keep < PERFLIBS_AMD_NUM_VARIANTS) A->rebuild_variant = keep;
 }
 
 // Value update (src/matvec.cpp:530-562; CSR search of

@@ -33,6 +33,7 @@ namespace pls {
 // formats this library holds: BSR expands with zero dropping (nnz reset),
 // SELL goes back to its source CSR with zeros dropped (scs2csr), CSR stays.
 perflibs_status_t convert_to_csr(perflibs_spmat_top_t *A) {
+  A->rebuild_variant = -1;  // a variant kept for the old format or op does not carry over
   if (A->fmt == Fmt::Bsr) {
     HostCsr c;
     int32_t nnz = 0;
@@ -82,7 +83,9 @@ perflibs_status_t ensure_plan(perflibs_spmat_top_t *A, int op) {
   if (!make_operand(A, op, o)) return exec_fail(A, "matrix format has no GPU SpMV path");
   if (A->fmt == Fmt::Scs) return build_sell_plan(A, o, op, A->scs_C, A->scs_sigma, 100);
   if (A->forced_C > 0) return build_sell_plan(A, o, op, A->forced_C, A->forced_sigma, 100);
-  int v = A->forced_variant >= 0 ? A->forced_variant : PERFLIBS_AMD_CSR_VECTOR;
+  int v = A->forced_variant >= 0   ? A->forced_variant
+          : A->rebuild_variant >= 0 ? A->rebuild_variant
+                                    : PERFLIBS_AMD_CSR_VECTOR;
   if (!variant_applicable(A, o, v)) v = PERFLIBS_AMD_CSR_VECTOR;
   return build_plan(A, o, op, v);
 }
@@ -92,6 +95,7 @@ perflibs_status_t ensure_plan(perflibs_spmat_top_t *A, int op) {
 perflibs_status_t spmv_optimize_impl(perflibs_spmat_top_t *A) {
   A->select_path = PERFLIBS_AMD_PATH_NONE;
   A->select_timed = 0;
+  A->rebuild_variant = -1;
   if (A->fmt == Fmt::Null || A->fmt == Fmt::Identity || A->fmt == Fmt::Dense)
     return PERFLIBS_STATUS_SUCCESS;
   if (A->fmt == Fmt::Csr && A->nnz == 0) return PERFLIBS_STATUS_SUCCESS;

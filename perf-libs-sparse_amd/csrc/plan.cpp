@@ -283,6 +283,14 @@ hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
   return e;
 }
 
+// The transposed operand's structure of a TRANS / CONJTRANS plan of a CSR or
+// SELL object, kept to locate value updates in place (update.cpp).
+static void keep_transposed_structure(perflibs_spmat_top_t *A, const Operand &o, int op) {
+  if (op == PERFLIBS_SPARSE_OPERATION_NOTRANS || !(A->fmt == Fmt::Csr || A->fmt == Fmt::Scs)) return;
+  A->plan.h_trp.assign(o.prp, o.prp + size_t(o.m) + 1);
+  A->plan.h_tci.assign(o.pci, o.pci + (int64_t(o.prp[o.m]) - o.prp[0]));
+}
+
 perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int op, int32_t C,
                                   int32_t sigma, int variant_tag) {
   SellHost s;
@@ -319,6 +327,7 @@ perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int
   A->plan.sell = std::move(d);
   A->plan.variant = variant_tag;
   A->plan.op = op;
+  keep_transposed_structure(A, o, op);
   return PERFLIBS_STATUS_SUCCESS;
 }
 
@@ -441,12 +450,7 @@ static perflibs_status_t build_plan_layout(perflibs_spmat_top_t *A, const Operan
 
 perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op, int variant) {
   const perflibs_status_t st = build_plan_layout(A, o, op, variant);
-  if (st == PERFLIBS_STATUS_SUCCESS && op != PERFLIBS_SPARSE_OPERATION_NOTRANS &&
-      (A->fmt == Fmt::Csr || A->fmt == Fmt::Scs)) {
-    // the transposed operand's structure, to locate value updates in place
-    A->plan.h_trp.assign(o.prp, o.prp + size_t(o.m) + 1);
-    A->plan.h_tci.assign(o.pci, o.pci + (int64_t(o.prp[o.m]) - o.prp[0]));
-  }
+  if (st == PERFLIBS_STATUS_SUCCESS && A->plan.h_trp.empty()) keep_transposed_structure(A, o, op);
   return st;
 }
 

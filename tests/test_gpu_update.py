@@ -256,3 +256,33 @@ def test_update_then_spmm_column_compressed(pls, orc, gen, k):
     torch.cuda.synchronize()
     v_new = host_update(rp, ci, v, rows, cols, vals)
     assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v_new, Bv)) <= 1e-12
+
+
+def test_transposed_forced_sell_plan_is_patched(pls, orc):
+    """A SELL-C-sigma plan forced with force_sell for op(A) = A^T keeps the
+    transposed operand's structure like every other TRANS plan, so an update
+    patches its device layout in place (the plan survives) instead of dropping
+    it and rebuilding the SELL layout on the host."""
+    rng = np.random.default_rng(91)
+    m, n = 2048, 3072
+    rp, ci, v, dup_rows = matrix_with_duplicates(rng, m, n)
+    x = rng.uniform(-1, 1, m)
+    A = pls.SpMat.csr(m, n, rp, ci, v)
+    A.hint(pls.HINT_SPMV_OPERATION, pls.TRANS)
+    A.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
+    A.force_sell(32, 64)
+    A.optimize()
+    y = np.zeros(n)
+    A.spmv(x, y, trans=pls.TRANS)
+    before = A.variant()
+    assert before >= 0
+    rows, cols, vals = pick_updates(rng, rp, ci, 80, np.array(dup_rows))
+    A.update(rows, cols, vals)
+    assert A.variant() == before, "the transposed SELL plan was dropped instead of patched"
+    y[:] = 0
+    A.spmv(x, y, trans=pls.TRANS)
+    v_new = host_update(rp, ci, v, rows, cols, vals)
+    D = np.zeros((m, n))
+    for i in range(m):
+        np.add.at(D[i], ci[rp[i]:rp[i + 1]], v_new[rp[i]:rp[i + 1]])
+    assert rel_err(y, D.T @ x) <= 1e-12
