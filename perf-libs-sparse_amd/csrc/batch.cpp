@@ -81,8 +81,9 @@ perflibs_status_t batch_fail(perflibs_status_t st, const std::string &msg) {
 }
 
 // Row tiles of at most tile_nnz non-zeros (a longer row is a tile of its own)
-// and at most 1024 rows (so a tile of up to 2048 non-zeros is staged in LDS by
-// the kernel), sorted by estimated bytes moved, largest first.
+// and at most plk::batch_staged_rows() rows (512 for FP64, 1024 for FP32
+// accumulation), so a tile of up to plk::kBatchStage non-zeros is staged in
+// LDS by the kernel; sorted by estimated bytes moved, largest first.
 void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<BatchTile> &tiles) {
   struct Costed {
     double cost;
@@ -91,6 +92,7 @@ void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<Ba
   std::vector<Costed> all;
   const double sv = B.vs == VStore::F64 ? 8 : B.vs == VStore::F32 ? 4 : 2;
   const double sx = B.dt == DType::F64 ? 8 : 4;
+  const int32_t row_cap = plk::batch_staged_rows(B.dt == DType::F64);
   for (int64_t p = 0; p < B.nprob; ++p) {
     const BatchProblem &pb = B.probs[size_t(p)];
     const int32_t *rp = B.h_rp.data() + pb.rp_off;
@@ -98,7 +100,7 @@ void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<Ba
     while (r < pb.m) {
       const int32_t r0 = r;
       int64_t nz = 0;
-      while (r < pb.m && r - r0 < 1024) {
+      while (r < pb.m && r - r0 < row_cap) {
         const int64_t len = rp[r + 1] - rp[r];
         if (r > r0 && nz + len > tile_nnz) break;
         nz += len;

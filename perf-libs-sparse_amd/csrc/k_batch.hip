@@ -19,14 +19,11 @@
 
 namespace plk {
 
-// A tile's column indices, alpha-scaled values and row offsets are staged in
-// LDS when they fit (at most kBatchStage non-zeros and kBatchRows<A> rows), so
-// the per-row loops only wait for the X rows they gather; a tile that does not
-// fit (a row longer than kBatchStage) reads the matrix from HBM directly.
-constexpr int kBatchStage = 2048;
-// Staged rows: 512 for FP64 tiles (26 KB of LDS, so six workgroups share a
-// CU), 1024 for FP32 (20 KB; more resident workgroups measured slower there).
-template <class A> constexpr int kBatchRows = sizeof(A) == 8 ? 512 : 1024;
+// Staged tiles (k_batch.hpp: at most kBatchStage non-zeros and
+// batch_staged_rows() rows) let the per-row loops wait only for the X rows they
+// gather; a tile that does not fit (a row longer than kBatchStage) reads the
+// matrix from HBM directly.
+template <class A> constexpr int kBatchRows = batch_staged_rows(sizeof(A) == 8);
 
 template <class A> struct BatchLds {
   int32_t ci[kBatchStage];

@@ -10,6 +10,14 @@
 
 namespace plk {
 
+// A tile's column indices, alpha-scaled values and row offsets are staged in
+// LDS when the tile has at most kBatchStage non-zeros and batch_staged_rows()
+// rows: 512 for FP64 tiles (26 KB of LDS, so six workgroups share a CU), 1024
+// for FP32 accumulation (FP32 and BF16 storage: 20 KB; more resident
+// workgroups measured slower there).  The host cuts tiles to these limits.
+constexpr int kBatchStage = 2048;
+constexpr int batch_staged_rows(bool f64) { return f64 ? 512 : 1024; }
+
 // k classes of the batch: k = 1, 2, 4, 8, 16 right-hand sides.
 constexpr int kBatchClasses = 5;
 
