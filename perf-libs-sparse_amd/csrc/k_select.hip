@@ -9,8 +9,9 @@
 
 namespace plk {
 
-constexpr int kStats = 8;  // sum len, sum len^2, max len, empty rows,
-                           // sum spread, max |col-row|, 4x4 blocks, 8x8 blocks
+constexpr int kStats = kRowStats;  // sum len, sum len^2, max len, empty rows, sum
+                                   // spread, max |col-row|, 4x4 blocks, 8x8 blocks,
+                                   // rows whose columns are not ascending
 
 __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci,
@@ -18,7 +19,7 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *
   __shared__ double red[kStats][kBlock];
   const int tid = threadIdx.x;
   const int64_t r = int64_t(blockIdx.x) * kBlock + tid;
-  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (r < m) {
     const int32_t s = rp[r], e = rp[r + 1], len = e - s;
     v[0] = len;
@@ -26,13 +27,18 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(int32_t m, const int32_t *
     v[2] = len;
     v[3] = len == 0;
     if (len > 0) {
-      int32_t mn = INT32_MAX, mx = INT32_MIN;
+      int32_t mn = INT32_MAX, mx = INT32_MIN, prev = INT32_MIN;
+      bool desc = false;
       for (int32_t j = s; j < e; ++j) {
-        mn = min(mn, ci[j]);
-        mx = max(mx, ci[j]);
+        const int32_t c = ci[j];
+        mn = min(mn, c);
+        mx = max(mx, c);
+        desc |= c < prev;
+        prev = c;
       }
       v[4] = double(mx) - mn + 1;
       v[5] = fmax(fabs(double(mx) - double(r)), fabs(double(mn) - double(r)));
+      v[8] = desc;
     }
   }
 #pragma unroll
@@ -61,15 +67,15 @@ __device__ __forceinline__ int32_t lower_bound_i32(const int32_t *a, int32_t lo,
   return lo;
 }
 
-// Distinct B-wide column blocks of every block row (the block-fill features),
-// one wave per block row: lane l merges only the entries whose block column
-// lies in its slice [l*S, (l+1)*S) of the nbc block columns, found by binary
-// search in each row, so a block row of long rows is split over the wave
-// (waves stride over the block rows).  Rows must have ascending columns
-// (unsorted rows over-count).  Writes one kStats record per workgroup with the
-// count in `slot`.
+// Distinct B-wide column blocks of every `stride`-th block row (the
+// block-fill features), one wave per block row: lane l merges only the entries
+// whose block column lies in its slice [l*S, (l+1)*S) of the nbc block columns,
+// found by binary search in each row, so a block row of long rows is split
+// over the wave (waves stride over the block rows).  Rows must have ascending
+// columns: the host passes a row-sorted copy when k_row_stats finds a row that
+// is not.  Writes one kStats record per workgroup with the count in `slot`.
 template <int B>
-__global__ __launch_bounds__(kBlock) void k_block_fill(int32_t m, int32_t nbc,
+__global__ __launch_bounds__(kBlock) void k_block_fill(int32_t m, int32_t nbc, int32_t stride,
                                                        const int32_t *__restrict__ rp,
                                                        const int32_t *__restrict__ ci,
                                                        double *__restrict__ partial, int slot) {
@@ -79,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_block_fill(int32_t m, int32_t nbc,
   const int32_t lo = lane * S, hi = min(nbc, lo + S);
   int64_t count = 0;
   const int64_t nbr = (int64_t(m) + B - 1) / B;
-  for (int64_t rb = int64_t(blockIdx.x) * (kBlock / 64) + wave; rb < nbr;
-       rb += int64_t(gridDim.x) * (kBlock / 64)) {
+  for (int64_t rb = (int64_t(blockIdx.x) * (kBlock / 64) + wave) * stride; rb < nbr;
+       rb += int64_t(gridDim.x) * (kBlock / 64) * stride) {
     if (lo >= hi) continue;
     int32_t pos[B], end[B];
 #pragma unroll
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_stats(int64_t nparts, const dou
                                                        double *out) {
   __shared__ double red[kStats][kBlock];
   const int tid = threadIdx.x;
-  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double v[kStats] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t p = tid; p < nparts; p += kBlock)
 #pragma unroll
     for (int k = 0; k < kStats; ++k) {
@@ -141,21 +147,26 @@ __global__ __launch_bounds__(kBlock) void k_fold_stats(int64_t nparts, const dou
   if (tid < kStats) out[tid] = red[tid][0];
 }
 
-hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s) {
-  const unsigned grid = grid_for(A.m);
-  if (grid == 0) return hipMemsetAsync(stats_dev, 0, sizeof(double) * kStats, s);
+hipError_t csr_row_stats(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci,
+                         int32_t stride4, int32_t stride8, bool with_rows, double *stats_dev,
+                         hipStream_t s) {
+  const unsigned grid = with_rows ? grid_for(m) : 0;
+  if (m == 0) return hipMemsetAsync(stats_dev, 0, sizeof(double) * kStats, s);
   constexpr int kWaves = kBlock / 64;  // block rows per workgroup of the fill kernels
-  const unsigned g4 = unsigned(std::min<int64_t>(2048, (int64_t(A.m) + 4 * kWaves - 1) / (4 * kWaves)));
-  const unsigned g8 = unsigned(std::min<int64_t>(2048, (int64_t(A.m) + 8 * kWaves - 1) / (8 * kWaves)));
+  auto fill_grid = [&](int b, int32_t stride) {
+    const int64_t rows = ((int64_t(m) + b - 1) / b + stride - 1) / stride;
+    return unsigned(std::min<int64_t>(2048, (rows + kWaves - 1) / kWaves));
+  };
+  const unsigned g4 = fill_grid(4, stride4), g8 = fill_grid(8, stride8);
   pls::DevBuf part;  // kStats records: row statistics, then 4x4 and 8x8 block counts
   hipError_t e = part.alloc(sizeof(double) * kStats * (size_t(grid) + g4 + g8));
   if (e != hipSuccess) return e;
   double *p = part.as<double>();
-  k_row_stats<<<grid, kBlock, 0, s>>>(A.m, A.rp.as<int32_t>(), A.ci.as<int32_t>(), p);
-  k_block_fill<4><<<g4, kBlock, 0, s>>>(A.m, int32_t((int64_t(A.n) + 3) / 4), A.rp.as<int32_t>(),
-                                        A.ci.as<int32_t>(), p + size_t(kStats) * grid, 6);
-  k_block_fill<8><<<g8, kBlock, 0, s>>>(A.m, int32_t((int64_t(A.n) + 7) / 8), A.rp.as<int32_t>(),
-                                        A.ci.as<int32_t>(), p + size_t(kStats) * (grid + g4), 7);
+  if (grid > 0) k_row_stats<<<grid, kBlock, 0, s>>>(m, rp, ci, p);
+  k_block_fill<4><<<g4, kBlock, 0, s>>>(m, int32_t((int64_t(n) + 3) / 4), stride4, rp, ci,
+                                        p + size_t(kStats) * grid, 6);
+  k_block_fill<8><<<g8, kBlock, 0, s>>>(m, int32_t((int64_t(n) + 7) / 8), stride8, rp, ci,
+                                        p + size_t(kStats) * (grid + g4), 7);
   k_fold_stats<<<1, kBlock, 0, s>>>(int64_t(grid) + g4 + g8, p, stats_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -232,11 +232,15 @@ struct perflibs_spmat_top_t {
   int select_path = -1;   // how the last optimize() chose (perflibs_amd_select_path)
   int select_timed = 0;   // variants it timed
   int forced_variant = -1;
+  // variant of a plan an update dropped (its layout could not be patched): the
+  // next execution rebuilds it; never read by selection, cleared by optimize
+  int rebuild_variant = -1;
   int forced_C = -1, forced_sigma = -1;
   int forced_panel_log2 = -1, forced_panel_wpc = -1;  // column-panel shape overrides
   int precision = PERFLIBS_AMD_VALUES_NATIVE;
 
   pls::DevBuf stage_x, stage_y;  // staging of host x / y
+  pls::DevBuf stage_b;           // op(B) packed row-major for the few-column SpMM
 };
 
 namespace pls {
@@ -352,6 +356,8 @@ perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A);
 // Column-compressed row blocks of the wide-B SpMM kernel for the CSR plan `d`
 // of operand `o` (sets d.lb_state; a HIP error is returned as is).
 hipError_t build_spmm_blocks(const Operand &o, DevCsr &d);
+// The device plan an SpMV with `op` runs (built on demand, src/matvec.cpp:122-129).
+perflibs_status_t ensure_plan(perflibs_spmat_top_t *A, int op);
 // Launch the current plan: y = alpha*A*x + beta*y on device vectors.
 perflibs_status_t launch_plan(perflibs_spmat_top_t *A, double alpha,
                               const void *x, double beta, void *y);
@@ -423,6 +429,12 @@ hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha,
 hipError_t scale_dense(bool dbl, int64_t m, int64_t n, double beta, void *C,
                        int64_t sr, int64_t sc, hipStream_t s);
 // dst[i*ds] = src[i*ss], i < n.
+// C = alpha*A*B + beta*C for n <= 4 columns, A read once: B row-major with
+// rows of few_rhs_pitch(n) values (16/32-byte aligned pieces), C through its
+// row and column strides.
+int few_rhs_pitch(int32_t n);
+hipError_t csr_spmm_few(const pls::DevCsr &A, bool dbl, int32_t n, double alpha, const void *B,
+                        double beta, void *C, int64_t c_sr, int64_t c_sc, hipStream_t s);
 hipError_t strided_copy(bool dbl, int64_t n, const void *src, int64_t ss, void *dst, int64_t ds,
                         hipStream_t s);
 // FP32 -> BF16 (round to nearest even), n values, device to device.
@@ -438,9 +450,16 @@ hipError_t patch_vals(VStore vs, void *vals, const int64_t *slot, const void *v,
 bool sell_widths_supported(int32_t C, int32_t sigma);
 hipError_t sell_widths(int32_t m, const int32_t *rp_dev, int32_t C, int32_t sigma,
                        int32_t *cl_dev, int64_t *cmn_dev, hipStream_t s);
-// Row statistics of a 0-based device CSR for the selector features:
-// out[0..] partial sums reduced on the device, see select.cpp.
-hipError_t csr_row_stats(const pls::DevCsr &A, double *stats_dev, hipStream_t s);
+// Statistics of a 0-based device CSR for the selector features, reduced on
+// the device into stats_dev[9]: sum of row lengths, sum of their squares, max
+// length, empty rows, sum of row column spans, max |column - row|, distinct
+// 4x4 and 8x8 blocks of every stride4-th / stride8-th block row (rows with
+// ascending columns assumed), rows whose columns are not ascending.  With
+// with_rows false only the block counts are computed (the other slots read 0).
+constexpr int kRowStats = 9;
+hipError_t csr_row_stats(int32_t m, int32_t n, const int32_t *rp, const int32_t *ci,
+                         int32_t stride4, int32_t stride8, bool with_rows, double *stats_dev,
+                         hipStream_t s);
 // MLP of the selector (I inputs, H hidden, K outputs), FP32 device buffers.
 hipError_t mlp_forward(const float *P, const float *X, int n, int I, int H, int K, float *h1,
                        float *h2, float *out, hipStream_t s);

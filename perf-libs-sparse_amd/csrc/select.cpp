@@ -47,6 +47,7 @@ struct Model {
   float fmin[kF] = {0}, fmax[kF] = {0};
 };
 constexpr int kAudition = 3;     // predictions timed in learned mode
+constexpr int64_t kFillSampleRows = 65536;  // block rows merged per fill feature, at most about
 constexpr float kZmax = 4.0f;    // without a stored range
 constexpr float kRangeSlack = 0.1f;  // share of a feature's training range accepted beyond it
 Model g_model;
@@ -125,12 +126,47 @@ perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, fl
     if (st != PERFLIBS_STATUS_SUCCESS) return st;
   }
   const DevCsr *csr = A->plan.csr.get();
+  // block fill from every stride-th block row: at most kFillSampleRows block
+  // rows are merged, so the fill features cost a bounded time on big operands
+  const int64_t nbr4 = (int64_t(o.m) + 3) / 4, nbr8 = (int64_t(o.m) + 7) / 8;
+  const int32_t stride4 = int32_t(std::max<int64_t>(1, nbr4 / kFillSampleRows));
+  const int32_t stride8 = int32_t(std::max<int64_t>(1, nbr8 / kFillSampleRows));
   DevBuf stats;
-  hipError_t e = stats.alloc(8 * sizeof(double));
-  if (e == hipSuccess) e = plk::csr_row_stats(*csr, stats.as<double>(), stream());
-  double st[8] = {0};
+  hipError_t e = stats.alloc(plk::kRowStats * sizeof(double));
+  if (e == hipSuccess)
+    e = plk::csr_row_stats(csr->m, o.n, csr->rp.as<int32_t>(), csr->ci.as<int32_t>(), stride4,
+                           stride8, true, stats.as<double>(), stream());
+  double st[plk::kRowStats] = {0};
   if (e == hipSuccess) e = hipMemcpy(st, stats.get(), sizeof(st), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(A, e, "matrix statistics");
+  if (st[8] > 0) {  // rows with unsorted columns: count blocks on a row-sorted copy
+    const int32_t base = o.prp[0];
+    std::vector<int32_t> sorted(size_t(int64_t(o.prp[o.m]) - base));
+    for (int32_t r = 0; r < o.m; ++r) {
+      const int64_t s0 = int64_t(o.prp[r]) - base, s1 = int64_t(o.prp[r + 1]) - base;
+      for (int64_t j = s0; j < s1; ++j) sorted[size_t(j)] = o.pci[j] - base;
+      std::sort(sorted.begin() + s0, sorted.begin() + s1);
+    }
+    DevBuf dci;
+    double sb[plk::kRowStats] = {0};
+    e = dci.upload(sorted.data(), sizeof(int32_t) * std::max<size_t>(sorted.size(), 1));
+    if (e == hipSuccess)
+      e = plk::csr_row_stats(csr->m, o.n, csr->rp.as<int32_t>(), dci.as<int32_t>(), stride4,
+                             stride8, false, stats.as<double>(), stream());
+    if (e == hipSuccess) e = hipMemcpy(sb, stats.get(), sizeof(sb), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(A, e, "block fill of sorted rows");
+    st[6] = sb[6];
+    st[7] = sb[7];
+  }
+  // non-zeros of the block rows the fill counts cover
+  auto sampled_nnz = [&](int b, int32_t stride, int64_t nbr) {
+    int64_t z = 0;
+    for (int64_t rb = 0; rb < nbr; rb += stride)
+      z += int64_t(o.prp[std::min<int64_t>((rb + 1) * b, o.m)]) - o.prp[rb * b];
+    return double(z);
+  };
+  const double nnz4 = stride4 == 1 ? st[0] : sampled_nnz(4, stride4, nbr4);
+  const double nnz8 = stride8 == 1 ? st[0] : sampled_nnz(8, stride8, nbr8);
   const double m = std::max(1, o.m), n = std::max(1, o.n);
   const double nnz = st[0];
   const double mean = nnz / m;
@@ -146,8 +182,8 @@ perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o, fl
   feat[6] = float(mean > 0 ? std::log(mx / mean) : 0.0);
   feat[7] = float(st[5] / n);
   feat[8] = float(st[4] / nonempty / n);
-  feat[9] = float(st[6] > 0 ? nnz / (st[6] * 16.0) : 0.0);
-  feat[10] = float(st[7] > 0 ? nnz / (st[7] * 64.0) : 0.0);
+  feat[9] = float(st[6] > 0 ? nnz4 / (st[6] * 16.0) : 0.0);
+  feat[10] = float(st[7] > 0 ? nnz8 / (st[7] * 64.0) : 0.0);
   feat[11] = float(st[3] / m);
   o.fill4 = feat[9];
   o.fill8 = feat[10];

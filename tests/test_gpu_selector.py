@@ -235,3 +235,36 @@ def test_block_fill_features_match_numpy(pls):
     for b, idx in ((4, 9), (8, 10)):
         blocks = len(np.unique((rows // b).astype(np.int64) * (n // b + 1) + ci // b))
         assert abs(f[idx] - len(ci) / (blocks * b * b)) <= 1e-6 * f[idx]
+
+
+def test_block_fill_features_unsorted_rows(pls):
+    """Rows whose columns are not ascending (the reference accepts them) give
+    the same block-fill features as the sorted matrix: the fill kernels run on
+    a row-sorted copy when any row is unsorted."""
+    rng = np.random.default_rng(32)
+    m, n = 4096, 3000
+    lens = rng.integers(0, 30, m)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    rows = [np.sort(rng.choice(n, int(l), replace=False)) for l in lens]
+    ci_sorted = np.concatenate(rows).astype(np.int32)
+    ci_shuffled = np.concatenate([rng.permutation(r) for r in rows]).astype(np.int32)
+    fs = pls.SpMat.csr(m, n, rp, ci_sorted, np.ones(len(ci_sorted))).features()
+    fu = pls.SpMat.csr(m, n, rp, ci_shuffled, np.ones(len(ci_sorted))).features()
+    assert np.array_equal(fs, fu)
+
+
+def test_block_fill_features_sampled_block_rows(pls, gen):
+    """On operands with more than 65536 block rows the fill features merge every
+    stride-th block row (stride = block rows // 65536) and divide the non-zeros
+    of exactly those block rows by their distinct blocks."""
+    m, n = 1_200_000, 50_000  # 300k / 150k block rows: strides 4 and 2
+    rp, ci, v = gen.random_rows(m, n, 2, 9, 0x5EED0041)
+    f = pls.SpMat.csr(m, n, rp, ci, v).features()
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(rp))
+    for b, idx in ((4, 9), (8, 10)):
+        nbr = (m + b - 1) // b
+        stride = max(1, nbr // 65536)
+        keep = (rows // b) % stride == 0
+        blocks = len(np.unique((rows[keep] // b) * (n // b + 1) + ci[keep] // b))
+        want = keep.sum() / (blocks * b * b)
+        assert stride > 1 and abs(f[idx] - want) <= 1e-6 * want, (b, f[idx], want)
