@@ -382,7 +382,7 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits);
 int pick_variant(perflibs_spmat_top_t *A, const Operand &o);
 perflibs_status_t compute_features(perflibs_spmat_top_t *A, const Operand &o,
                                    float *feat);
-int heuristic_variant(const float *feat, const Operand &o);
+int heuristic_variant(const perflibs_spmat_top_t *A, const float *feat, const Operand &o);
 // Times the candidate variants (only[v] false: skipped, only == nullptr: all)
 // with warmup + reps launches each; ms[v] < 0 for variants not timed.
 perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op,

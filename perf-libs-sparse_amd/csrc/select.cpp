@@ -243,15 +243,17 @@ perflibs_status_t sell_beta_of(perflibs_spmat_top_t *A, const Operand &o, const 
 
 // Hand rules on the same features: dense blocks -> BSR, columns scattered
 // over an x much larger than an XCD's L2 -> column panels, strongly irregular
-// rows -> CSR-adaptive, otherwise SELL-C-sigma with a moderate sort window.
-int heuristic_variant(const float *f, const Operand &o) {
+// rows -> CSR-adaptive, otherwise SELL-C-sigma with a moderate sort window
+// when that layout passes the padding rule, else CSR-adaptive.
+int heuristic_variant(const perflibs_spmat_top_t *A, const float *f, const Operand &o) {
   const double fill4 = f[9], fill8 = f[10], imbalance = f[6], mean = std::expm1(f[3]);
   const double spread = f[8];  // mean column span of a row / n
   if (fill8 >= 0.9 && o.m % 8 == 0 && o.n % 8 == 0) return PERFLIBS_AMD_BSR_B8;
   if (fill4 >= 0.9 && o.m % 4 == 0 && o.n % 4 == 0) return PERFLIBS_AMD_BSR_B4;
   if (o.n >= (1 << 20) && o.m >= (1 << 18) && spread > 0.25) return PERFLIBS_AMD_CSR_PANEL;
   if (imbalance > std::log(8.0)) return PERFLIBS_AMD_CSR_ADAPTIVE;
-  return mean >= 24 ? PERFLIBS_AMD_SELL_C64_S1024 : PERFLIBS_AMD_SELL_C32_S1024;
+  const int sell = mean >= 24 ? PERFLIBS_AMD_SELL_C64_S1024 : PERFLIBS_AMD_SELL_C32_S1024;
+  return variant_candidate(A, o, sell) ? sell : PERFLIBS_AMD_CSR_ADAPTIVE;
 }
 
 perflibs_status_t time_variants(perflibs_spmat_top_t *A, const Operand &o, int op, int warmup,
@@ -348,7 +350,7 @@ static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const floa
   if (e == hipSuccess) e = hipMemcpy(pred, out.get(), sizeof(pred), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     A->select_path = PERFLIBS_AMD_PATH_HEURISTIC;
-    return heuristic_variant(f, o);
+    return heuristic_variant(A, f, o);
   }
   int order[kK], nc = 0;
   for (int v = 0; v < kK; ++v)
@@ -393,7 +395,7 @@ int pick_variant(perflibs_spmat_top_t *A, const Operand &o) {
   if ((mode == PERFLIBS_AMD_SELECT_LEARNED || mode == PERFLIBS_AMD_SELECT_NETWORK) && g_model.loaded)
     return learned_variant(A, o, f, mode == PERFLIBS_AMD_SELECT_LEARNED);
   A->select_path = PERFLIBS_AMD_PATH_HEURISTIC;
-  return heuristic_variant(f, o);
+  return heuristic_variant(A, f, o);
 }
 
 }  // namespace pls
