@@ -198,8 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   // a thread stages
   constexpr int NIF = UMAX / NG < 8 ? UMAX / NG : 8;
   __shared__ __align__(16) A sB[UMAX * KC];
-  __shared__ A s_a[pls::kLbNnzPerUcol * UMAX];
-  __shared__ uint8_t s_l[pls::kLbNnzPerUcol * UMAX];
+  __shared__ A s_a[pls::lb_nnz_per_ucol(UMAX) * UMAX];
+  __shared__ uint8_t s_l[pls::lb_nnz_per_ucol(UMAX) * UMAX];
   __shared__ int32_t s_rp[pls::kLbRows + 1];
   __shared__ int32_t s_uc[UMAX];
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);

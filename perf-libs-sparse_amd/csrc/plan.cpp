@@ -200,7 +200,7 @@ static perflibs_status_t build_csr_plan(perflibs_spmat_top_t *A, const Operand &
 // kernels, which read B for every non-zero, are used.
 namespace {
 hipError_t cut_spmm_blocks(const Operand &o, DevCsr &d, int32_t kLbUcols) {
-  const int32_t kLbNnz = kLbNnzPerUcol * kLbUcols;
+  const int32_t kLbNnz = lb_nnz_per_ucol(kLbUcols) * kLbUcols;
   const int32_t m = o.m;
   const int32_t *rp = o.prp, *ci = o.pci;
   const int32_t base = rp[0];

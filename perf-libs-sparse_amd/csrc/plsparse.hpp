@@ -124,10 +124,14 @@ struct DevCsr {
 };
 // Limits of a column-compressed SpMM block (must match k_spmm.hip): distinct
 // columns per list (128, or 256 when lists of 128 leave too many rows without
-// a list; positions fit a byte), non-zeros (4 per list column) and rows.
+// a list; positions fit a byte), non-zeros and rows.  Non-zeros per list
+// column: 4 for lists of 128 (the LDS then leaves room for six workgroups per
+// CU on the 27-point stencil; 8 measured 1.2x slower there), 8 for lists of
+// 256, so a whole 8 x 8 block row (8 rows of 216 non-zeros under 216 columns)
+// is one block (8 x 8-block operator, k = 32: 6.5 -> 5.0 ms).
 constexpr int32_t kLbUcolsSmall = 128;
 constexpr int32_t kLbUcolsLarge = 256;
-constexpr int32_t kLbNnzPerUcol = 4;
+constexpr int32_t lb_nnz_per_ucol(int32_t umax) { return umax > kLbUcolsSmall ? 8 : 4; }
 constexpr int32_t kLbRows = 128;
 
 // Device SELL-C-sigma (layout of src/sell_c_sigma.hpp:43-104, NOTRANS).
