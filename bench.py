@@ -720,10 +720,10 @@ def main() -> int:
         torch.cuda.empty_cache()
         extras["laplace2d_316"] = laplace_line(50)
         t0 = time.perf_counter()
-        extras["spmm_stencil27_128"] = spmm_lines(10, 2, False)
+        extras["spmm_stencil27_128"] = spmm_lines(10, 2, not args.no_cpu)
         log(f"spmm lines in {time.perf_counter() - t0:.1f}s")
         t0 = time.perf_counter()
-        extras["batched_banded_4096"] = batch_lines(args.batch, 10, 2, False)
+        extras["batched_banded_4096"] = batch_lines(args.batch, 10, 2, not args.no_cpu)
         log(f"batch lines in {time.perf_counter() - t0:.1f}s")
 
     out = {
