@@ -115,8 +115,10 @@ perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t pa
 /* Column-compressed row blocks of the SpMM plan (for tests and tuning):
  * blocks_state 0 = not built, 1 = built and used for k >= 8, -1 = built but
  * too little B-row reuse (direct kernels run); list_max = the list length
- * limit the blocks were cut with; list_total = total list length.  Status 1
- * when the handle has no CSR plan yet. */
+ * limit the blocks were cut with; list_total = total list length.
+ * blocks_state 2: NOTRANS products run on the blocks of a former BSR object
+ * (nblocks = block rows, list_max = block size, list_total = blocks).
+ * Status 1 when the handle has neither. */
 perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blocks_state,
                                          perflibs_int_t *nblocks, perflibs_int_t *list_max,
                                          int64_t *list_total);

@@ -564,6 +564,7 @@ perflibs_status_t update(perflibs_spmat_t A, perflibs_int_t nu, const perflibs_i
   if (!A) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   if (nu < 0) return pls::param_fail(A, 1, "The number of updates can not be negative.");
   if (A->dt != dtype_of<T>()) return pls::param_fail(A, 1, "datatype mismatch");
+  A->spmm_bsr.reset();  // its values are about to change
   const int32_t base = A->index_base;
   std::vector<pls::ValueHit> hits;
   if (A->fmt == Fmt::Csr || A->fmt == Fmt::Scs) {

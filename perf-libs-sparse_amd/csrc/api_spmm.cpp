@@ -132,6 +132,21 @@ template <class T> perflibs_status_t relayout(perflibs_spmat_top_t *D, int layou
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+// The SpMM conversion of A to CSR (src/matmul.cpp:619-631: a BSR object is
+// expanded, explicit zeros dropped).  With `keep` a BSR object with 4 x 4 or
+// 8 x 8 blocks first leaves its blocks on the device for the native product.
+perflibs_status_t to_csr_keeping_blocks(perflibs_spmat_top_t *A, bool keep) {
+  if (A->fmt == Fmt::Csr) return PERFLIBS_STATUS_SUCCESS;
+  std::unique_ptr<pls::DevBsr> blocks;
+  if (keep && A->fmt == Fmt::Bsr) {
+    const perflibs_status_t st = pls::bsr_blocks_for_spmm(A, blocks);
+    if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  }
+  pls::convert_to_csr(A);
+  if (blocks) A->spmm_bsr = std::move(blocks);
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 // CSR plan of op(A) for SpMM with k columns of B; for k >= 8 also the
 // column-compressed row blocks of the wide-B kernel (built once per plan).
 perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op, int64_t k) {
@@ -308,13 +323,24 @@ perflibs_status_t spmm(int transA, int transB, T alpha, perflibs_spmat_top_t *A,
   perflibs_status_t st = dense_to_dev(B);
   if (st == PERFLIBS_STATUS_SUCCESS) st = dense_to_dev(C);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
-  if (A->fmt != Fmt::Csr) pls::convert_to_csr(A);
+  st = to_csr_keeping_blocks(A, !ta);
+  if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  pls::Dense &cd = C->dense;
+  const pls::Dense &bd = B->dense;
+  if (!ta && A->spmm_bsr) {  // the product on the blocks of the former BSR object
+    if (!exec) return PERFLIBS_STATUS_SUCCESS;
+    e = plk::bsr_spmm(*A->spmm_bsr, dbl, n, double(alpha), bd.dev.get(), bd.lda, double(beta),
+                      cd.dev.get(), cd.lda, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM");
+    cd.host_valid = false;
+    cd.matrix_is_zero = false;
+    return PERFLIBS_STATUS_SUCCESS;
+  }
   st = csr_plan_for(A, transA, n);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   if (!exec) return PERFLIBS_STATUS_SUCCESS;
 
-  pls::Dense &cd = C->dense;
-  const pls::Dense &bd = B->dense;
   // op(B) row-major view: NOTRANS row-major B, or TRANS column-major B
   e = plk::csr_spmm(*A->plan.csr, dbl, n, double(alpha), bd.dev.get(), bd.lda, 1, double(beta),
                     cd.dev.get(), cd.lda, 1, s);
@@ -333,8 +359,15 @@ perflibs_status_t spmm_async(T alpha, perflibs_spmat_top_t *A, perflibs_int_t k,
   if (A->dt != dt || special(A) || A->fmt == Fmt::Dense) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   hipError_t e = pls::ensure_device();
   if (e != hipSuccess) return pls::hip_fail(A, e, "no usable HIP device");
-  if (A->fmt != Fmt::Csr) pls::convert_to_csr(A);
-  perflibs_status_t st = csr_plan_for(A, PERFLIBS_SPARSE_OPERATION_NOTRANS, k);
+  perflibs_status_t st = to_csr_keeping_blocks(A, true);
+  if (st != PERFLIBS_STATUS_SUCCESS) return st;
+  if (A->spmm_bsr) {
+    e = plk::bsr_spmm(*A->spmm_bsr, dt == DType::F64, k, double(alpha), B, ldb, double(beta), C,
+                      ldc, pls::stream());
+    if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM");
+    return PERFLIBS_STATUS_SUCCESS;
+  }
+  st = csr_plan_for(A, PERFLIBS_SPARSE_OPERATION_NOTRANS, k);
   if (st != PERFLIBS_STATUS_SUCCESS) return st;
   e = plk::csr_spmm(*A->plan.csr, dt == DType::F64, k, double(alpha), B, ldb, 1, double(beta), C,
                     ldc, 1, pls::stream());

@@ -294,7 +294,15 @@ perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t pa
 perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blocks_state,
                                          perflibs_int_t *nblocks, perflibs_int_t *list_max,
                                          int64_t *list_total) {
-  if (!A || !A->plan.csr) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (!A) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  if (A->spmm_bsr) {  // NOTRANS products run on the former BSR object's blocks
+    if (blocks_state) *blocks_state = 2;
+    if (nblocks) *nblocks = A->spmm_bsr->mb;
+    if (list_max) *list_max = A->spmm_bsr->b;
+    if (list_total) *list_total = A->spmm_bsr->nnzb;
+    return PERFLIBS_STATUS_SUCCESS;
+  }
+  if (!A->plan.csr) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   const pls::DevCsr &d = *A->plan.csr;
   if (blocks_state) *blocks_state = d.lb_state;
   if (nblocks) *nblocks = d.lb_nblocks;
@@ -359,7 +367,10 @@ perflibs_status_t perflibs_amd_set_precision(perflibs_spmat_t A, int precision) 
     return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
   if (precision == PERFLIBS_AMD_VALUES_BF16 && A->dt != DType::F32)
     return pls::param_fail(A, 2, "BF16 value storage needs a single-precision (_s) handle");
-  if (precision != A->precision) A->plan.clear();
+  if (precision != A->precision) {
+    A->plan.clear();
+    A->spmm_bsr.reset();
+  }
   A->precision = precision;
   return PERFLIBS_STATUS_SUCCESS;
 }

@@ -393,6 +393,83 @@ __global__ __launch_bounds__(kBlock) void k_csr_few_rhs(
   }
 }
 
+// Native BSR x dense (b = 4 or 8, row-major B and C): a wave holds 64 / (b * G)
+// block rows and a column window of each.  Lane (i, cl) of a block row owns row
+// i and CPL adjacent columns c0 + cl * CPL + t (G column lanes per row); for
+// each block the lanes of its block row load the b x b values with one
+// coalesced load into the wave's LDS slice and each lane reads its row's b
+// values back, and the b rows of B the block names are read once per block
+// row (the b lanes that share a column read the same address), so A streams
+// at its 8 bytes per value and every B row is reused by the block's b rows.
+// A stored zero adds nothing (0 * Inf stays out of the sum), matching the
+// product of the zero-dropped CSR expansion the reference computes
+// (src/matmul.cpp:619-649, src/compressed_sparse_rows.cpp:966-1056).
+template <int BS, int G, int CPL, bool VEC, class V, class X, bool BETA, bool COLMAJ>
+__global__ __launch_bounds__(kBlock) void k_bsr_spmm(
+    int32_t mb, int32_t npass, int32_t k, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const V *__restrict__ val, const X *__restrict__ Bm,
+    int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  constexpr int BB = BS * BS;      // values per block
+  constexpr int GL = BS * G;       // lanes of one block row
+  constexpr int RPW = 64 / GL;     // block rows per wave
+  static_assert(GL >= BB && RPW >= 1, "a block row's lanes load its blocks in one pass");
+  __shared__ A s_blk[kBlock / 64][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = lane / GL, i = (lane % GL) / G, cl = lane % G;
+  const int64_t w = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * (kBlock / 64) + wave;
+  const int64_t unit = w * RPW + sub;  // (block row, column window) of this lane
+  const bool live = unit < int64_t(mb) * npass;
+  const int32_t br = live ? int32_t(unit / npass) : 0;
+  const int32_t c0 = live ? int32_t(unit % npass) * (G * CPL) + cl * CPL : k;
+  const int32_t p0 = live ? rp[br] : 0, p1 = live ? rp[br + 1] : 0;
+  // every lane of the wave runs the longest block row's count of steps
+  int32_t steps = p1 - p0;
+#pragma unroll
+  for (int o = GL; o < 64; o <<= 1) steps = max(steps, __shfl_xor(steps, o));
+  A acc[CPL];
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) acc[t] = A(0);
+  A *blk = s_blk[wave] + sub * GL;
+  for (int32_t q = 0; q < steps; ++q) {
+    const bool on = p0 + q < p1;
+    const int32_t p = on ? p0 + q : p0;
+    const int32_t bc = on ? ci[p] : 0;
+    blk[lane % GL] = on ? A(ldv(val, int64_t(p) * BB + ((lane % GL) % BB))) : A(0);
+    __builtin_amdgcn_wave_barrier();
+    A a[BS];
+#pragma unroll
+    for (int jj = 0; jj < BS; ++jj) a[jj] = blk[COLMAJ ? jj * BS + i : i * BS + jj];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int jj = 0; jj < BS; ++jj) {
+      const X *b = Bm + (int64_t(bc) * BS + jj) * ldb + c0;
+      A bv[CPL];
+      if constexpr (VEC) {
+        if (on) Piece<CPL, X>::load(b, bv);
+        else
+#pragma unroll
+          for (int t = 0; t < CPL; ++t) bv[t] = A(0);
+      } else {
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) bv[t] = on && c0 + t < k ? A(b[t]) : A(0);
+      }
+#pragma unroll
+      for (int t = 0; t < CPL; ++t) acc[t] += a[jj] != A(0) ? a[jj] * bv[t] : A(0);
+    }
+  }
+  if (!live) return;
+  X *c = Cm + (int64_t(br) * BS + i) * ldc + c0;
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    if (c0 + t < k) {
+      A r = alpha * acc[t];
+      if (BETA) r += beta * A(c[t]);
+      c[t] = X(r);
+    }
+  }
+}
+
 template <class X>
 __global__ __launch_bounds__(kBlock) void k_scale_dense(int64_t m, int64_t n, acc_t<X> beta,
                                                         X *Cm, int64_t sr, int64_t sc) {
@@ -577,6 +654,76 @@ hipError_t csr_spmm_few(const pls::DevCsr &A, bool dbl, int32_t n, double alpha,
       case 3: few_lanes<3, 4, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
       default: few_lanes<4, 4, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
     }
+    return hipGetLastError();
+  };
+  if (dbl) {
+    if (A.vs != pls::VStore::F64) return hipErrorInvalidValue;
+    return run((const double *)nullptr, (double *)nullptr);
+  }
+  if (A.vs == pls::VStore::F32) return run((const float *)nullptr, (float *)nullptr);
+  if (A.vs == pls::VStore::BF16) return run((const uint16_t *)nullptr, (float *)nullptr);
+  return hipErrorInvalidValue;
+}
+
+namespace {
+template <int BS, int G, int CPL, class V, class X>
+void launch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
+                     double beta, void *C, int64_t ldc, hipStream_t s) {
+  constexpr int WCOLS = G * CPL;          // columns of one block row's window
+  constexpr int RPW = 64 / (BS * G);      // block rows per wave
+  const int32_t npass = (k + WCOLS - 1) / WCOLS;
+  const int64_t units = int64_t(A.mb) * npass;
+  const int64_t waves = (units + RPW - 1) / RPW;
+  const unsigned grid = unsigned((waves + kBlock / 64 - 1) / (kBlock / 64));
+  if (grid == 0) return;
+  // vector loads of CPL columns: every piece inside a row and aligned
+  const bool vec = CPL > 1 && k % WCOLS == 0 && ldb % CPL == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % (CPL * sizeof(X)) == 0;
+  const bool cm = A.layout == PERFLIBS_COL_MAJOR;
+  auto go = [&](auto kern) {
+    kern<<<grid, kBlock, 0, s>>>(A.mb, npass, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
+                                 A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha),
+                                 acc_t<X>(beta));
+  };
+  auto by_layout = [&](auto vtag, auto btag) {
+    constexpr bool VEC = decltype(vtag)::value, BETA = decltype(btag)::value;
+    if (cm) go(k_bsr_spmm<BS, G, CPL, VEC, V, X, BETA, true>);
+    else go(k_bsr_spmm<BS, G, CPL, VEC, V, X, BETA, false>);
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  if (vec) beta != 0.0 ? by_layout(T_{}, T_{}) : by_layout(T_{}, F_{});
+  else beta != 0.0 ? by_layout(F_{}, T_{}) : by_layout(F_{}, F_{});
+}
+// Window shapes from k: 8 x 8 blocks keep 8 column lanes per row (one block
+// row per wave) with 1, 2 or 4 columns per lane; 4 x 4 blocks take 4, 8 or 16
+// column lanes (4, 2 or 1 block rows per wave), then 2 or 4 columns per lane.
+template <int BS, class V, class X>
+void dispatch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
+                       double beta, void *C, int64_t ldc, hipStream_t s) {
+  if constexpr (BS == 8) {
+    if (k <= 8) launch_bsr_spmm<8, 8, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else if (k <= 16) launch_bsr_spmm<8, 8, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else launch_bsr_spmm<8, 8, 4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+  } else {
+    if (k <= 4) launch_bsr_spmm<4, 4, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else if (k <= 8) launch_bsr_spmm<4, 8, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else if (k <= 16) launch_bsr_spmm<4, 16, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else if (k <= 32) launch_bsr_spmm<4, 16, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else launch_bsr_spmm<4, 16, 4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+  }
+}
+}  // namespace
+
+hipError_t bsr_spmm(const pls::DevBsr &A, bool dbl, int32_t k, double alpha, const void *B,
+                    int64_t ldb, double beta, void *C, int64_t ldc, hipStream_t s) {
+  if (A.mb == 0 || k == 0) return hipSuccess;
+  if (!(A.b == 4 || A.b == 8)) return hipErrorInvalidValue;
+  auto run = [&](auto vtag, auto xtag) -> hipError_t {
+    using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
+    using X = std::remove_pointer_t<decltype(xtag)>;
+    if (A.b == 8) dispatch_bsr_spmm<8, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else dispatch_bsr_spmm<4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
     return hipGetLastError();
   };
   if (dbl) {

@@ -331,10 +331,11 @@ perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-static perflibs_status_t upload_bsr(perflibs_spmat_top_t *A, int layout, int32_t b, int32_t mb,
-                                    int32_t nb, const int32_t *rp, const int32_t *ci,
-                                    const void *vals, int op, int variant,
-                                    std::unique_ptr<DevBsr> d = std::make_unique<DevBsr>()) {
+// Device BSR blocks (0-based) of a block structure, values in the handle's
+// storage type.
+static hipError_t fill_dev_bsr(const perflibs_spmat_top_t *A, int layout, int32_t b, int32_t mb,
+                               int32_t nb, const int32_t *rp, const int32_t *ci,
+                               const void *vals, DevBsr *d) {
   d->layout = layout;
   d->b = b;
   d->mb = mb;
@@ -353,13 +354,32 @@ static perflibs_status_t upload_bsr(perflibs_spmat_top_t *A, int layout, int32_t
     uci = ci0.data();
   }
   hipError_t e = d->rp.upload(urp, sizeof(int32_t) * (size_t(mb) + 1));
-  if (e == hipSuccess) e = d->ci.upload(uci, sizeof(int32_t) * size_t(d->nnzb));
+  if (e == hipSuccess) e = d->ci.upload(uci, sizeof(int32_t) * size_t(std::max<int64_t>(d->nnzb, 1)));
   if (e == hipSuccess) e = upload_vals(d->vals, vals, d->nnzb * b * b, A->dt, d->vs);
+  return e;
+}
+
+static perflibs_status_t upload_bsr(perflibs_spmat_top_t *A, int layout, int32_t b, int32_t mb,
+                                    int32_t nb, const int32_t *rp, const int32_t *ci,
+                                    const void *vals, int op, int variant,
+                                    std::unique_ptr<DevBsr> d = std::make_unique<DevBsr>()) {
+  const hipError_t e = fill_dev_bsr(A, layout, b, mb, nb, rp, ci, vals, d.get());
   if (e != hipSuccess) return hip_fail(A, e, "BSR plan upload");
   A->plan.clear();
   A->plan.bsr = std::move(d);
   A->plan.variant = variant;
   A->plan.op = op;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t bsr_blocks_for_spmm(perflibs_spmat_top_t *A, std::unique_ptr<DevBsr> &out) {
+  out.reset();
+  const HostBsr &B = A->bsr;
+  if (A->fmt != Fmt::Bsr || !(B.b == 4 || B.b == 8) || B.mb == 0) return PERFLIBS_STATUS_SUCCESS;
+  auto d = std::make_unique<DevBsr>();
+  const hipError_t e = fill_dev_bsr(A, B.layout, B.b, B.mb, B.nb, B.rp, B.ci, B.vals, d.get());
+  if (e != hipSuccess) return hip_fail(A, e, "BSR blocks upload");
+  out = std::move(d);
   return PERFLIBS_STATUS_SUCCESS;
 }
 

@@ -245,6 +245,11 @@ struct perflibs_spmat_top_t {
 
   pls::DevBuf stage_x, stage_y;  // staging of host x / y
   pls::DevBuf stage_b;           // op(B) packed row-major for the few-column SpMM
+  // The blocks of a BSR object kept on the device when an SpMM converts it to
+  // CSR (as the reference does, src/matmul.cpp:619-631): NOTRANS products run
+  // on them natively while the values are those of the blocks (dropped by a
+  // value update or a precision change).
+  std::unique_ptr<pls::DevBsr> spmm_bsr;
 };
 
 namespace pls {
@@ -357,6 +362,9 @@ perflibs_status_t build_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
 perflibs_status_t build_sell_plan(perflibs_spmat_top_t *A, const Operand &o, int op,
                                   int32_t C, int32_t sigma, int variant_tag);
 perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A);
+// The blocks of a BSR object with b = 4 or 8, uploaded for the native BSR x
+// dense product (out stays empty for any other object).
+perflibs_status_t bsr_blocks_for_spmm(perflibs_spmat_top_t *A, std::unique_ptr<DevBsr> &out);
 // Column-compressed row blocks of the wide-B SpMM kernel for the CSR plan `d`
 // of operand `o` (sets d.lb_state; a HIP error is returned as is).
 hipError_t build_spmm_blocks(const Operand &o, DevCsr &d);
@@ -433,6 +441,10 @@ hipError_t csr_spmm(const pls::DevCsr &A, bool dbl, int32_t k, double alpha,
 hipError_t scale_dense(bool dbl, int64_t m, int64_t n, double beta, void *C,
                        int64_t sr, int64_t sc, hipStream_t s);
 // dst[i*ds] = src[i*ss], i < n.
+// C = alpha*A*B + beta*C on BSR blocks (b = 4 or 8), B and C row-major with
+// leading dimensions ldb and ldc; stored zeros add nothing.
+hipError_t bsr_spmm(const pls::DevBsr &A, bool dbl, int32_t k, double alpha, const void *B,
+                    int64_t ldb, double beta, void *C, int64_t ldc, hipStream_t s);
 // C = alpha*A*B + beta*C for n <= 4 columns, A read once: B row-major with
 // rows of few_rhs_pitch(n) values (16/32-byte aligned pieces), C through its
 // row and column strides.

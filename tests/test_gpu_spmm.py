@@ -378,10 +378,11 @@ def test_spmm_lists_of_256_values(pls, orc, gen, dtype, k, beta):
 
 
 @pytest.mark.parametrize("k", [8, 32])
-def test_spmm_bsr8_drop_in_lists_of_256(pls, orc, gen, k):
-    """The drop-in perflibs_spmm_exec_d on an 8 x 8 BSR handle (expanded like
-    the reference's bsr2csr SpMM path, src/matmul.cpp:619-649) runs the lists
-    of 256 and matches the oracle's bsr2csr + product."""
+def test_spmm_bsr8_drop_in_native_blocks(pls, orc, gen, k):
+    """The drop-in perflibs_spmm_exec_d on an 8 x 8 BSR handle converts the
+    object to CSR like the reference's SpMM path (src/matmul.cpp:619-649) and
+    runs the product on the kept blocks; it matches the oracle's bsr2csr +
+    product."""
     m, brp, bci, bv, _, _, _ = block8_operand(gen)
     rng = np.random.default_rng(21 + k)
     Bv = rng.uniform(-1, 1, (m, k))
@@ -394,7 +395,8 @@ def test_spmm_bsr8_drop_in_lists_of_256(pls, orc, gen, k):
     rp, ci, v = orc.bsr2csr(91, m, m, 8, brp, bci, bv)
     assert rel_err(Cm.export_dense(), orc.spmm(m, k, rp, ci, v, Bv, -0.75, 0.5, C0)) <= TOL64
     info = A.spmm_info()
-    assert info["blocks_state"] == 1 and info["list_max"] == 256
+    assert info["blocks_state"] == 2 and info["list_max"] == 8
+    assert A.query()[3] == len(ci)  # the object is CSR now, explicit zeros dropped
 
 
 @pytest.mark.parametrize("k", [8, 40])
@@ -479,3 +481,82 @@ def test_spmm_few_columns(pls, orc, gen, k, layout, dtype):
     Cs = dense(C0)
     pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.25, As, B, 0.7, Cs)
     assert rel_err(Cs.export_dense(), orc.spmm(m, k, rp, ci, vd, Bv, 1.25, 0.7, C0)) <= tol
+
+
+def block_operand(gen, b, layout, seed):
+    """Block-stencil operator with dense N(0,1) b x b blocks stored in `layout`;
+    every stored block of block column bci[0] is all zero, so the b rows of B
+    that column names meet stored zeros only."""
+    brp, bci, bv = gen.block_stencil(6, 5, 4, b, 0x5EED0005 + seed)
+    bv = bv.copy().reshape(len(bci), b * b)
+    bv[bci == bci[0]] = 0.0
+    bv = bv.reshape(-1)
+    if layout == "col":
+        nb = len(bci)
+        bv = bv.reshape(nb, b, b).transpose(0, 2, 1).copy().reshape(-1)
+    return brp, bci, bv
+
+
+@pytest.mark.parametrize("b", [4, 8])
+@pytest.mark.parametrize("layout", ["row", "col"])
+@pytest.mark.parametrize("k", [5, 13, 32, 64, 100])
+def test_spmm_native_bsr(pls, orc, gen, b, layout, k):
+    """BSR x dense on the blocks (b = 4, 8; row- and column-major blocks),
+    FP64 and FP32, beta 0 and 0.5, widths that take the vector and the
+    per-element loads, against the oracle's bsr2csr + product; a stored zero
+    block times Inf in B adds nothing (the CSR expansion drops it)."""
+    lay = pls.ROW_MAJOR if layout == "row" else pls.COL_MAJOR
+    brp, bci, bv = block_operand(gen, b, layout, k)
+    m = (len(brp) - 1) * b
+    rp, ci, v = orc.bsr2csr(lay, m, m, b, brp, bci, bv)
+    rng = np.random.default_rng(100 + k)
+    for dtype, tol in ((np.float64, TOL64), (np.float32, 1e-6)):
+        Bv = rng.uniform(-1, 1, (m, k)).astype(dtype).astype(np.float64)
+        # B rows that only stored zeros meet get an Inf
+        zero_rows = np.arange(bci[0] * b, bci[0] * b + b)
+        assert not np.isin(zero_rows, ci).any()
+        Bv[zero_rows, 0] = np.inf
+        C0 = rng.uniform(-1, 1, (m, k)).astype(dtype).astype(np.float64)
+        for beta in (0.0, 0.5):
+            A = pls.SpMat.bsr(lay, m, m, b, brp, bci, bv.astype(dtype))
+            B = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, Bv.astype(dtype).reshape(-1))
+            Cm = pls.SpMat.dense(pls.ROW_MAJOR, m, k, k, C0.astype(dtype).reshape(-1))
+            pls.spmm_exec(pls.NOTRANS, pls.NOTRANS, 1.5, A, B, beta, Cm)
+            got = Cm.export_dense()
+            assert np.isfinite(got).all()
+            Bf = np.where(np.isinf(Bv), 0.0, Bv)
+            ref = orc.spmm(m, k, rp, ci, v.astype(dtype).astype(np.float64), Bf, 1.5, beta, C0)
+            assert rel_err(got, ref) <= tol, (dtype, beta)
+            assert A.spmm_info()["blocks_state"] == 2
+
+
+def test_spmm_native_bsr_async_update_and_precision(pls, orc, gen):
+    """The asynchronous SpMM of a BSR handle runs on its blocks too; a value
+    update or a precision change drops them (the CSR path then runs) and the
+    results follow the updated values."""
+    import torch
+
+    brp, bci, bv = block_operand(gen, 8, "row", 3)
+    m = (len(brp) - 1) * 8
+    k = 24
+    rng = np.random.default_rng(5)
+    Bv = rng.uniform(-1, 1, (m, k))
+    A = pls.SpMat.bsr(pls.ROW_MAJOR, m, m, 8, brp, bci, bv)
+    Bd = torch.from_numpy(Bv).cuda()
+    Cd = torch.zeros((m, k), dtype=torch.float64, device="cuda")
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 2.0, 0.0)
+    torch.cuda.synchronize()
+    rp, ci, v = orc.bsr2csr(pls.ROW_MAJOR, m, m, 8, brp, bci, bv)
+    assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v, Bv, 2.0)) <= TOL64
+    assert A.spmm_info()["blocks_state"] == 2
+    # update two stored entries of the (now CSR) object
+    rows = np.array([10, 200], dtype=np.int32)
+    cols = np.array([ci[rp[10]], ci[rp[200] + 3]], dtype=np.int32)
+    A.update(rows, cols, np.array([7.0, -3.0]))
+    v2 = v.copy()
+    v2[rp[10]] = 7.0
+    v2[rp[200] + 3] = -3.0
+    A.spmm_async(k, Bd.data_ptr(), k, Cd.data_ptr(), k, 2.0, 0.0)
+    torch.cuda.synchronize()
+    assert A.spmm_info()["blocks_state"] != 2
+    assert rel_err(Cd.cpu().numpy(), orc.spmm(m, k, rp, ci, v2, Bv, 2.0)) <= TOL64
