@@ -114,25 +114,54 @@ def measure(m, n, rp, ci, v, warmup=3, reps=10):
     return feat, ms, heur
 
 
-def collect(count: int, seed: int, out: Path, log=print):
-    feats, times, heur, kinds, sizes = [], [], [], [], []
+def matrix_seed(run_seed: int, i: int) -> int:
+    """Generator seed of matrix i of a collect run (sample_matrix's `seed`)."""
+    return run_seed * 100_003 + i
+
+
+def collect(count: int, seed: int, out: Path, log=print, max_nnz: int = MAX_NNZ):
+    feats, times, heur, kinds, sizes, seeds = [], [], [], [], [], []
     t0 = time.time()
     for i in range(count):
         kind = GENERATORS[i % len(GENERATORS)]
-        m, n, rp, ci, v, desc = sample_matrix(kind, seed * 100_003 + i)
+        ms_seed = matrix_seed(seed, i)
+        m, n, rp, ci, v, desc = sample_matrix(kind, ms_seed, max_nnz=max_nnz)
         f, ms, h = measure(m, n, rp, ci, v)
         feats.append(f)
         times.append(ms)
         heur.append(h)
         kinds.append(kind)
         sizes.append((m, int(rp[-1])))
+        seeds.append(ms_seed)
         best = int(np.argmax(np.where(ms > 0, -ms, -np.inf)))
         log(f"[collect] {i + 1}/{count} {desc} m={m} nnz={int(rp[-1])} best={VARIANTS[best]} "
             f"{ms[best]:.4f} ms heuristic={VARIANTS[h]} ({time.time() - t0:.0f}s)")
         if (i + 1) % 20 == 0 or i + 1 == count:  # keep what was measured so far
             np.savez(out, features=np.array(feats, np.float32), times=np.array(times, np.float64),
                      heuristic=np.array(heur, np.int32), kinds=np.array(kinds),
-                     sizes=np.array(sizes, np.int64))
+                     sizes=np.array(sizes, np.int64), seeds=np.array(seeds, np.int64),
+                     max_nnz=np.full(len(seeds), max_nnz, np.int64))
+
+
+def merge(parts, out: Path, legacy=()):
+    """Concatenate collect outputs.  legacy: (file, run seed, max_nnz) for files
+    written before seeds were recorded, whose matrix i has seed
+    matrix_seed(run seed, i)."""
+    legacy = {str(f): (s, mx) for f, s, mx in legacy}
+    cols = {k: [] for k in ("features", "times", "heuristic", "kinds", "sizes", "seeds", "max_nnz")}
+    for f in parts:
+        d = np.load(f)
+        n = len(d["times"])
+        for k in ("features", "times", "heuristic", "kinds", "sizes"):
+            cols[k].append(d[k])
+        if "seeds" in d:
+            cols["seeds"].append(d["seeds"])
+            cols["max_nnz"].append(d["max_nnz"])
+        else:
+            rs, mx = legacy[str(f)]
+            cols["seeds"].append(np.array([matrix_seed(rs, i) for i in range(n)], np.int64))
+            cols["max_nnz"].append(np.full(n, mx, np.int64))
+    np.savez(out, **{k: np.concatenate(v) for k, v in cols.items()})
 
 
 def targets(times: np.ndarray):
@@ -270,6 +299,13 @@ def main(argv=None) -> int:
     c.add_argument("--out", required=True)
     c.add_argument("--count", type=int, default=300)
     c.add_argument("--seed", type=int, default=1)
+    c.add_argument("--max-nnz", type=int, default=MAX_NNZ,
+                   help="non-zeros per matrix at most (rows are cut to fit)")
+    mg = sub.add_parser("merge")
+    mg.add_argument("--out", required=True)
+    mg.add_argument("parts", nargs="+")
+    mg.add_argument("--legacy", action="append", default=[],
+                    help="FILE:RUN_SEED:MAX_NNZ for a part written without per-matrix seeds")
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--model", default=str(DEFAULT_MODEL))
@@ -288,7 +324,13 @@ def main(argv=None) -> int:
     if args.cmd == "collect":
         import torch  # noqa: F401  (one HIP runtime: torch's)
 
-        collect(args.count, args.seed, Path(args.out), log)
+        collect(args.count, args.seed, Path(args.out), log, args.max_nnz)
+    elif args.cmd == "merge":
+        legacy = []
+        for item in args.legacy:
+            f, rs, mx = item.rsplit(":", 2)
+            legacy.append((f, int(rs), int(mx)))
+        merge(args.parts, Path(args.out), legacy)
     elif args.cmd == "train":
         d = np.load(args.data)
         feats, times = d["features"], d["times"]
@@ -316,9 +358,11 @@ def main(argv=None) -> int:
             check("load_selector_model", lib().perflibs_amd_load_selector_model(str(args.model).encode()))
             picks = picks_of(pred, d["times"])
             agree = 0
+            seeds = d["seeds"] if "seeds" in d else [matrix_seed(args.seed, i) for i in range(len(picks))]
+            caps = d["max_nnz"] if "max_nnz" in d else [MAX_NNZ] * len(picks)
             for i in range(args.end_to_end):
-                kind = GENERATORS[i % len(GENERATORS)]
-                m, n, rp, ci, v, desc = sample_matrix(kind, args.seed * 100_003 + i)
+                kind = str(d["kinds"][i])
+                m, n, rp, ci, v, desc = sample_matrix(kind, int(seeds[i]), max_nnz=int(caps[i]))
                 A = SpMat.csr(m, n, rp, ci, v, flags=CREATE_NOCOPY)
                 A.hint(HINT_SPMV_INVOCATIONS, INVOCATIONS_MANY)
                 A.select_mode(SELECT_LEARNED)

@@ -286,9 +286,11 @@ def test_learned_selector_regret_on_regenerated_heldout(pls):
     assert len(set(d["kinds"][order])) == len(S.GENERATORS)
     t0 = time.time()
     times, learned, heur = [], [], []
+    seeds = d["seeds"] if "seeds" in d else [S.matrix_seed(2, i) for i in range(len(d["times"]))]
+    caps = d["max_nnz"] if "max_nnz" in d else [S.MAX_NNZ] * len(d["times"])
     for i in order:
-        kind = S.GENERATORS[i % len(S.GENERATORS)]
-        m, n, rp, ci, v, desc = S.sample_matrix(kind, 2 * 100_003 + int(i))
+        kind = str(d["kinds"][i])
+        m, n, rp, ci, v, desc = S.sample_matrix(kind, int(seeds[i]), max_nnz=int(caps[i]))
         assert (m, int(rp[-1])) == tuple(d["sizes"][i]), (i, desc)
         A = pls.SpMat.csr(m, n, rp, ci, v, flags=pls.CREATE_NOCOPY)
         ms = A.time_variants(3, 10)
