@@ -30,6 +30,7 @@ NOTRANS, TRANS, CONJTRANS = 300, 310, 320
 INVOCATIONS_SINGLE, INVOCATIONS_FEW, INVOCATIONS_MANY = 400, 410, 420
 SCALAR_ONE, SCALAR_ZERO, SCALAR_ANY = 500, 501, 502
 CREATE_NOCOPY = 1
+NORM_INF, NORM_FRB = 1001, 1002
 
 # ---- perflibs_sparse_amd.h ----
 VARIANTS = [
@@ -91,6 +92,16 @@ _SIGS = {
     "perflibs_spmat_export_dense_s": (_I, [_P, C.c_int, _P, _P, _PP]),
     "perflibs_spmat_export_dense_d": (_I, [_P, C.c_int, _P, _P, _PP]),
     "perflibs_spmat_print_err": (None, [_P]),
+    "perflibs_spmat_export_csc_s": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_csc_d": (_I, [_P, _I, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_coo_s": (_I, [_P, _P, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spmat_export_coo_d": (_I, [_P, _P, _P, _P, _PP, _PP, _PP]),
+    "perflibs_spscale_exec_s": (_I, [_F, _P]),
+    "perflibs_spscale_exec_d": (_I, [_D, _P]),
+    "perflibs_sptranspose_exec_s": (_I, [C.c_int, _P]),
+    "perflibs_sptranspose_exec_d": (_I, [C.c_int, _P]),
+    "perflibs_spnorm_exec_s": (_I, [_P, C.c_int, _P]),
+    "perflibs_spnorm_exec_d": (_I, [_P, C.c_int, _P]),
     "perflibs_amd_set_stream": (_I, [_P]),
     "perflibs_amd_get_stream": (_P, []),
     "perflibs_amd_spmv_exec_async_d": (_I, [C.c_int, _D, _P, _P, _D, _P]),
@@ -375,6 +386,31 @@ class SpMat:
             check(fn, st)
         return st
 
+    # ---- in-place scaling, transposition, norms (host copies of the object) ----
+    def scale(self, alpha, raise_on_error=True) -> int:
+        fn = f"perflibs_spscale_exec_{self._sfx(self.dtype)}"
+        st = getattr(lib(), fn)(alpha, self.h)
+        if raise_on_error:
+            check(fn, st)
+        return st
+
+    def transpose(self, trans=TRANS, raise_on_error=True) -> int:
+        fn = f"perflibs_sptranspose_exec_{self._sfx(self.dtype)}"
+        st = getattr(lib(), fn)(trans, self.h)
+        if raise_on_error:
+            check(fn, st)
+        return st
+
+    def norm(self, kind=NORM_FRB, raise_on_error=True):
+        """The norm as a Python float, or (status, None) with raise_on_error=False on failure."""
+        fn = f"perflibs_spnorm_exec_{self._sfx(self.dtype)}"
+        out = (C.c_double if self.dtype == np.float64 else C.c_float)()
+        st = getattr(lib(), fn)(self.h, kind, C.byref(out))
+        if raise_on_error:
+            check(fn, st)
+            return float(out.value)
+        return (st, float(out.value) if st == 0 else None)
+
     # ---- export ----
     def export_csr(self, index_base=0):
         m, n = C.c_int32(), C.c_int32()
@@ -391,6 +427,38 @@ class SpMat:
         for p in (rp, ci, v):
             libc.free(p)
         return mm, n.value, rpa, cia, va
+
+    def export_csc(self, index_base=0):
+        """(m, n, row_indx, col_ptr, vals) of the object's CSC form."""
+        m, n = C.c_int32(), C.c_int32()
+        ri, cp, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        fn = f"perflibs_spmat_export_csc_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(self.h, index_base, C.byref(m), C.byref(n), C.byref(ri), C.byref(cp), C.byref(v)))
+        cpa = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_int32)), (n.value + 1,)).copy()
+        nnz = int(cpa[-1] - cpa[0])
+        ria = np.ctypeslib.as_array(C.cast(ri, C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy() if nnz else np.zeros(0, np.int32)
+        ctype = C.c_double if self.dtype == np.float64 else C.c_float
+        va = np.ctypeslib.as_array(C.cast(v, C.POINTER(ctype)), (max(nnz, 1),))[:nnz].copy() if nnz else np.zeros(0, self.dtype)
+        libc = C.CDLL(None)
+        for p in (ri, cp, v):
+            libc.free(p)
+        return m.value, n.value, ria, cpa, va
+
+    def export_coo(self):
+        """(m, n, row_indx, col_indx, vals) of the object's COO form."""
+        m, n, nnz = C.c_int32(), C.c_int32(), C.c_int32()
+        ri, ci, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        fn = f"perflibs_spmat_export_coo_{self._sfx(self.dtype)}"
+        check(fn, getattr(lib(), fn)(self.h, C.byref(m), C.byref(n), C.byref(nnz), C.byref(ri), C.byref(ci), C.byref(v)))
+        k = nnz.value
+        ctype = C.c_double if self.dtype == np.float64 else C.c_float
+        out = []
+        for p, ct, dt in ((ri, C.c_int32, np.int32), (ci, C.c_int32, np.int32), (v, ctype, self.dtype)):
+            out.append(np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), (k,)).copy() if k else np.zeros(0, dt))
+        libc = C.CDLL(None)
+        for p in (ri, ci, v):
+            libc.free(p)
+        return (m.value, n.value, *out)
 
     def export_bsr(self, block_layout=ROW_MAJOR, index_base=0):
         """(m, n, block_size, row_ptr, col_indx, vals[nnzb * b * b])."""
