@@ -192,9 +192,11 @@ def batch_lines(nprob: int, steps: int, warmup: int, with_cpu: bool) -> dict:
         sx = 8 if prec == "fp64" else 4
         nbytes = sum(int(rp[-1]) * (sv + 4) + (m + 1) * 4 + m * k * sx * 2 for (m, rp, _, _, k) in probs)
         info = B.info()
+        sell = B.sell_info()
         entry = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1), "algorithmic_bytes": nbytes,
                  "autotune_s": round(tune_s, 2), "tile_nnz": info["tile_nnz"], "tiles": info["ntiles"],
-                 "lanes_per_row": info["lanes"], "cols_per_lane": info["cpl"]}
+                 "lanes_per_row": info["lanes"], "cols_per_lane": info["cpl"],
+                 "k1_layout": f"sell_c{sell['C']}_s{sell['sigma']}" if sell["C"] else "csr_tiles"}
         ys = np.concatenate([y.double().cpu().numpy().ravel() for y in Y])
         if prec == "fp64":
             y64 = ys

@@ -286,6 +286,18 @@ perflibs_status_t perflibs_amd_batch_set_shape(perflibs_amd_batch_t B, perflibs_
  * from a hipGraph (captured on first use and after a change of alpha, beta or
  * shape).  Not synchronized. */
 perflibs_status_t perflibs_amd_batch_exec(perflibs_amd_batch_t B, double alpha, double beta);
+/* Run the k = 1 problems of the batch as SELL-C-sigma tiles: C = 32 or 64 rows
+ * per chunk (one lane per row), rows sorted by length inside sigma-row windows
+ * (sigma 0: unsorted, else a multiple of C), the layout of
+ * src/sell_c_sigma.cpp:145-336 per problem; C = 0 returns them to CSR row
+ * tiles.  Built on the host from the batch's device copy; optimize(autotune)
+ * also tries (0, 0), (32, 256), (64, 256) and (64, 1024) and keeps the fastest. */
+perflibs_status_t perflibs_amd_batch_set_sell(perflibs_amd_batch_t B, perflibs_int_t C,
+                                              perflibs_int_t sigma);
+/* Current SELL-C-sigma choice of the k = 1 problems (C 0: CSR row tiles) and
+ * the padded entries of their layout. */
+perflibs_status_t perflibs_amd_batch_sell_info(perflibs_amd_batch_t B, perflibs_int_t *C,
+                                               perflibs_int_t *sigma, int64_t *padded);
 /* Current shape: tiles, tile_nnz, resident workgroups, lanes[5], cpl[5]. */
 perflibs_status_t perflibs_amd_batch_info(perflibs_amd_batch_t B, int64_t *ntiles,
                                           perflibs_int_t *tile_nnz, perflibs_int_t *grid,

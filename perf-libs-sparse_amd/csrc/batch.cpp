@@ -8,14 +8,23 @@
 // X_p / Y_p stay caller-owned device arrays, reached through device tables of
 // pointers, so new operands need no new graph.
 //
+// The k = 1 problems can instead run as SELL-C-sigma tiles (C = 32 or 64,
+// rows sorted by length inside sigma-row windows, the per-problem layout of
+// sell_structure): a chunk table, columns, values and a slot -> row map, built
+// on the host from the device copy of those problems.
+//
 // exec launches one kernel over all tiles from a hipGraph captured once per
-// (alpha, beta, shape).  optimize(autotune) times candidate tile sizes and
-// per-k-class row shapes (lanes per row, columns per lane) by graph replays on
-// scratch outputs and keeps the fastest, one coordinate at a time.
+// (alpha, beta, shape).  optimize(autotune) times candidate tile sizes,
+// per-k-class row shapes (lanes per row, columns per lane) and the SELL-C-sigma
+// (C, sigma) of the k = 1 problems by graph replays on scratch outputs and
+// keeps the fastest, one coordinate at a time.
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "k_batch.hpp"
 #include "plsparse.hpp"
@@ -41,6 +50,12 @@ struct perflibs_amd_batch_s {
   BatchShape shape{};
   int32_t ntiles = 0, grid = 0;
   int32_t cus = 1, occ = 1;
+  // SELL-C-sigma layout of the k = 1 problems (sell_C 0: they run as CSR tiles)
+  int32_t sell_C = 0, sell_sigma = 0;
+  std::vector<plk::BatchSellChunk> h_chunks;
+  std::vector<int32_t> sell_first;  // per problem: first chunk (k = 1 problems)
+  std::vector<int32_t> sell_count;  // per problem: chunks
+  pls::DevBuf chunks_d, sci_d, svals_d, perm_d;
   // captured launch
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -95,6 +110,24 @@ void build_tiles(const perflibs_amd_batch_s &B, int32_t tile_nnz, std::vector<Ba
   const int32_t row_cap = plk::batch_staged_rows(B.dt == DType::F64);
   for (int64_t p = 0; p < B.nprob; ++p) {
     const BatchProblem &pb = B.probs[size_t(p)];
+    if (B.sell_C > 0 && pb.kclass == 0) {  // SELL-C-sigma chunks, tile_nnz padded slots per tile
+      const int32_t c_end = B.sell_first[size_t(p)] + B.sell_count[size_t(p)];
+      int32_t c = B.sell_first[size_t(p)];
+      while (c < c_end) {
+        const int32_t c0 = c;
+        int64_t slots = 0;
+        while (c < c_end) {
+          const int64_t w = int64_t(B.h_chunks[size_t(c)].width) * B.sell_C;
+          if (c > c0 && slots + w > tile_nnz) break;
+          slots += w;
+          ++c;
+        }
+        const double rows = double(c - c0) * B.sell_C;
+        const double cost = double(slots) * (sv + 4 + sx / 4) + rows * sx * 2;
+        all.push_back({cost, BatchTile{int32_t(p), c0, c, 1}});
+      }
+      continue;
+    }
     const int32_t *rp = B.h_rp.data() + pb.rp_off;
     int32_t r = 0;
     while (r < pb.m) {
@@ -146,7 +179,128 @@ plk::BatchLaunch launch_of(const perflibs_amd_batch_s *B, const void *xs, const 
   L.alpha = alpha;
   L.beta = beta;
   L.shape = B->shape;
+  L.sell_C = B->sell_C;
+  L.chunks = B->chunks_d.as<plk::BatchSellChunk>();
+  L.sci = B->sci_d.as<int32_t>();
+  L.svals = B->svals_d.get();
+  L.perm = B->perm_d.as<int32_t>();
   return L;
+}
+
+// SELL-C-sigma layout of the k = 1 problems (C 32 or 64, sigma 0 or a
+// multiple of C; C = 0 drops it): columns and values come back from the
+// batch's device copy, each problem is laid out by sell_structure, and the
+// chunks, columns, values and slot -> row map go to HBM.
+perflibs_status_t set_sell_layout(perflibs_amd_batch_s *B, int32_t C, int32_t sigma) {
+  hipError_t e = hipStreamSynchronize(stream());  // a running launch may read the old layout
+  if (e != hipSuccess) return batch_fail(PERFLIBS_STATUS_EXECUTION_FAILURE, "batch sync failed");
+  B->sell_C = 0;
+  B->sell_sigma = 0;
+  B->h_chunks.clear();
+  B->chunks_d.reset();
+  B->sci_d.reset();
+  B->svals_d.reset();
+  B->perm_d.reset();
+  B->graph_ok = false;
+  if (C == 0) return upload_tiles(B);
+  const size_t vsz = B->vs == VStore::F64 ? 8 : B->vs == VStore::F32 ? 4 : 2;
+  B->sell_first.assign(size_t(B->nprob), 0);
+  B->sell_count.assign(size_t(B->nprob), 0);
+  // pass 1: each k = 1 problem's columns and values back from HBM, its layout,
+  // and where its chunks, slots and padded entries start in the batch arrays
+  std::vector<int64_t> probs;
+  for (int64_t p = 0; p < B->nprob; ++p)
+    if (B->probs[size_t(p)].kclass == 0 && B->probs[size_t(p)].m > 0) probs.push_back(p);
+  std::vector<SellHost> lay(probs.size());
+  std::vector<std::vector<int32_t>> cis(probs.size());
+  std::vector<std::vector<uint8_t>> vls(probs.size());
+  std::vector<int64_t> off0(probs.size()), slot0(probs.size());
+  int64_t slots = 0, padded = 0, nchunks = 0;
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const int64_t p = probs[q];
+    const BatchProblem &pb = B->probs[size_t(p)];
+    const int32_t *rp = B->h_rp.data() + pb.rp_off;
+    const int64_t nz = rp[pb.m];
+    cis[q].resize(size_t(std::max<int64_t>(nz, 1)));
+    vls[q].resize(size_t(std::max<int64_t>(nz, 1)) * vsz);
+    if (nz) {
+      e = hipMemcpy(cis[q].data(), B->ci.as<int32_t>() + pb.nz_off, sizeof(int32_t) * size_t(nz),
+                    hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        e = hipMemcpy(vls[q].data(),
+                      static_cast<const uint8_t *>(B->vals.get()) + vsz * size_t(pb.nz_off),
+                      vsz * size_t(nz), hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return batch_fail(PERFLIBS_STATUS_EXECUTION_FAILURE, "batch download failed");
+    SellHost &sh = lay[q];
+    if (!sell_structure(pb.m, rp, cis[q].data(), C, sigma, sh))
+      return batch_fail(PERFLIBS_STATUS_INPUT_PARAMETER_ERROR, "batch SELL layout failed");
+    B->sell_first[size_t(p)] = int32_t(nchunks);
+    B->sell_count[size_t(p)] = int32_t(sh.nC);
+    off0[q] = padded;
+    slot0[q] = slots;
+    nchunks += sh.nC;
+    slots += sh.nC * C;
+    padded += sh.padded;
+    if (nchunks > INT32_MAX || slots > INT32_MAX)
+      return batch_fail(PERFLIBS_STATUS_INPUT_PARAMETER_ERROR,
+                        "batch SELL layout: more than 2^31 chunks or row slots");
+  }
+  // pass 2: the column-major slices, problems in parallel
+  B->h_chunks.resize(size_t(nchunks));
+  std::vector<int32_t> sci(size_t(std::max<int64_t>(padded, 1)), 0);
+  std::vector<uint8_t> sval(size_t(std::max<int64_t>(padded, 1)) * vsz, 0);
+  std::vector<int32_t> perm(size_t(std::max<int64_t>(slots, 1)), -1);
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t q = next++; q < probs.size(); q = next++) {
+      const BatchProblem &pb = B->probs[size_t(probs[q])];
+      const int32_t *rp = B->h_rp.data() + pb.rp_off;
+      const SellHost &sh = lay[q];
+      const int32_t first = B->sell_first[size_t(probs[q])];
+      for (int64_t c = 0; c < sh.nC; ++c) {
+        plk::BatchSellChunk &ch = B->h_chunks[size_t(first + c)];
+        ch.off = off0[q] + sh.cs[size_t(c)];
+        ch.width = int32_t(sh.cl[size_t(c)]);
+        ch.slot0 = int32_t(slot0[q] + c * C);
+        for (int32_t r = 0; r < C; ++r) {
+          const int64_t in = sh.permd2in[size_t(c * C + r)];
+          if (in < 0) continue;
+          perm[size_t(ch.slot0 + r)] = int32_t(in);
+          for (int32_t j = rp[in]; j < rp[in + 1]; ++j) {
+            const int64_t at = ch.off + int64_t(j - rp[in]) * C + r;
+            sci[size_t(at)] = cis[q][size_t(j)];
+            std::memcpy(&sval[size_t(at) * vsz], &vls[q][size_t(j) * vsz], vsz);
+          }
+        }
+      }
+    }
+  };
+  {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = std::min<size_t>(std::max(1u, std::min(hw, 16u)), std::max<size_t>(probs.size(), 1));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+  }
+  e = B->chunks_d.upload(B->h_chunks.data(),
+                         sizeof(plk::BatchSellChunk) * std::max<size_t>(B->h_chunks.size(), 1));
+  if (e == hipSuccess) e = B->sci_d.upload(sci.data(), sizeof(int32_t) * sci.size());
+  if (e == hipSuccess) e = B->svals_d.upload(sval.data(), sval.size());
+  if (e == hipSuccess) e = B->perm_d.upload(perm.data(), sizeof(int32_t) * perm.size());
+  if (e != hipSuccess) {
+    B->chunks_d.reset();
+    B->sci_d.reset();
+    B->svals_d.reset();
+    B->perm_d.reset();
+    B->h_chunks.clear();
+    (void)upload_tiles(B);
+    return batch_fail(PERFLIBS_STATUS_EXECUTION_FAILURE, "batch SELL upload failed");
+  }
+  B->sell_C = C;
+  B->sell_sigma = sigma;
+  return upload_tiles(B);
 }
 
 // Capture one launch into a new executable graph.
@@ -168,6 +322,11 @@ hipError_t capture(perflibs_amd_batch_s *B, const plk::BatchLaunch &L, hipGraph_
   }
   return e;
 }
+
+// (C, sigma) of the k = 1 problems tried by the autotuner; C = 0: CSR row tiles.
+// Sorting windows of 256 or 1024 rows keep the chunk widths near the rows'
+// own lengths (banded rows of 5-40 non-zeros pad ~1.75x unsorted at C = 64).
+constexpr int32_t kBatchSellCandidates[][2] = {{0, 0}, {32, 256}, {64, 256}, {64, 1024}};
 
 void default_shape(perflibs_amd_batch_s *B) {
   for (int c = 0; c < kBatchClasses; ++c) {
@@ -450,10 +609,56 @@ perflibs_status_t perflibs_amd_batch_optimize(perflibs_amd_batch_t B, int autotu
     B->shape.lanes[c] = bl;
     B->shape.cpl[c] = bc;
   }
+  // 3) the k = 1 problems as CSR row tiles or SELL-C-sigma chunk tiles
+  if (e == hipSuccess && B->has_class[0]) {
+    const int32_t keep_C = B->sell_C, keep_s = B->sell_sigma;
+    int32_t bC = keep_C, bs = keep_s;
+    for (const auto &cs : kBatchSellCandidates) {
+      if (e != hipSuccess) break;
+      if (cs[0] == keep_C && cs[1] == keep_s) continue;
+      if (set_sell_layout(B, cs[0], cs[1]) != PERFLIBS_STATUS_SUCCESS) {
+        e = hipErrorUnknown;
+        break;
+      }
+      double ms = 0;
+      e = time_current(ms);
+      if (e == hipSuccess && ms < best) {
+        best = ms;
+        bC = cs[0];
+        bs = cs[1];
+      }
+    }
+    if (e == hipSuccess && (B->sell_C != bC || B->sell_sigma != bs) &&
+        set_sell_layout(B, bC, bs) != PERFLIBS_STATUS_SUCCESS)
+      e = hipErrorUnknown;
+  }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   B->graph_ok = false;
   if (e != hipSuccess) return batch_fail(PERFLIBS_STATUS_EXECUTION_FAILURE, "batch autotune failed");
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_batch_set_sell(perflibs_amd_batch_t B, perflibs_int_t C,
+                                              perflibs_int_t sigma) {
+  if (!B) return batch_fail(PERFLIBS_STATUS_INPUT_PARAMETER_ERROR, "batch_set_sell: null batch");
+  if (!(C == 0 || C == 32 || C == 64) || sigma < 0 || sigma > (1 << 20) ||
+      (C > 0 && sigma % C != 0))
+    return batch_fail(PERFLIBS_STATUS_INPUT_PARAMETER_ERROR,
+                      "batch_set_sell: C must be 0, 32 or 64 and sigma 0 or a multiple of C");
+  return set_sell_layout(B, C, C == 0 ? 0 : sigma);
+}
+
+perflibs_status_t perflibs_amd_batch_sell_info(perflibs_amd_batch_t B, perflibs_int_t *C,
+                                               perflibs_int_t *sigma, int64_t *padded) {
+  if (!B) return batch_fail(PERFLIBS_STATUS_INPUT_PARAMETER_ERROR, "batch_sell_info: null batch");
+  if (C) *C = B->sell_C;
+  if (sigma) *sigma = B->sell_sigma;
+  if (padded) {
+    int64_t n = 0;
+    for (const auto &ch : B->h_chunks) n += int64_t(ch.width) * B->sell_C;
+    *padded = n;
+  }
   return PERFLIBS_STATUS_SUCCESS;
 }
 

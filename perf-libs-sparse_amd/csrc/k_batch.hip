@@ -121,6 +121,50 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
   }
 }
 
+// SELL-C-sigma tile of a k = 1 problem: C lanes per chunk (one row per lane),
+// 64 / C chunks per wave, every wave of the workgroup on its own chunks.  The
+// column / value slices of a chunk are contiguous across its lanes, so each
+// step of the row loop is one coalesced load per array; padding slots hold
+// value 0 and column 0 of the problem.  Four non-zeros in flight per lane.
+template <int C, class V, class X, bool BETA>
+__device__ __forceinline__ void batch_sell_tile(const BatchTile &tl,
+                                                const BatchSellChunk *__restrict__ chunks,
+                                                const int32_t *__restrict__ sci,
+                                                const V *__restrict__ sval,
+                                                const int32_t *__restrict__ perm,
+                                                const X *__restrict__ xp, X *__restrict__ yp,
+                                                acc_t<X> alpha, acc_t<X> beta) {
+  using A = acc_t<X>;
+  const int tid = threadIdx.x, r = tid % C;
+  for (int32_t c = tl.r0 + tid / C; c < tl.r1; c += kBlock / C) {
+    const BatchSellChunk ch = chunks[c];
+    A acc = 0;
+    if (alpha != A(0)) {
+      const int32_t *cc = sci + ch.off + r;
+      const V *vv = sval + ch.off + r;
+      int32_t j = 0;
+      for (; j + 4 <= ch.width; j += 4) {
+        int32_t col[4];
+        A a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          col[u] = ldi(cc, int64_t(j + u) * C);
+          a[u] = A(ldv(vv, int64_t(j + u) * C));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += a[u] * A(xp[col[u]]);
+      }
+      for (; j < ch.width; ++j) acc += A(ldv(vv, int64_t(j) * C)) * A(xp[ldi(cc, int64_t(j) * C)]);
+    }
+    const int32_t row = perm[ch.slot0 + r];
+    if (row >= 0) {
+      A res = alpha * acc;
+      if (BETA) res += beta * A(yp[row]);
+      yp[row] = X(res);
+    }
+  }
+}
+
 template <class V, class X, bool BETA>
 // At least six waves per SIMD: caps the FP64 kernel at 84 VGPRs (86 without),
 // which with 26 KB of LDS lets six workgroups share a CU (FP64 batch 7.93 ->
@@ -129,13 +173,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     int32_t ntiles, const BatchTile *__restrict__ tiles, const BatchProblem *__restrict__ probs,
     const int32_t *__restrict__ rp_all, const int32_t *__restrict__ ci_all,
     const V *__restrict__ val_all, const X *const *__restrict__ xs, X *const *__restrict__ ys,
-    acc_t<X> alpha, acc_t<X> beta, BatchShape shape) {
+    acc_t<X> alpha, acc_t<X> beta, BatchShape shape, int32_t sell_C,
+    const BatchSellChunk *__restrict__ chunks, const int32_t *__restrict__ sci,
+    const V *__restrict__ sval, const int32_t *__restrict__ perm) {
   __shared__ BatchLds<acc_t<X>> lds;
   for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchTile tl = tiles[t];
-    const BatchProblem pb = probs[tl.prob];
     const X *xp = xs[tl.prob];
     X *yp = ys[tl.prob];
+    if (tl.kind == 1) {  // no LDS: the next CSR tile's barrier does not wait on it
+      if (sell_C == 32)
+        batch_sell_tile<32, V, X, BETA>(tl, chunks, sci, sval, perm, xp, yp, alpha, beta);
+      else
+        batch_sell_tile<64, V, X, BETA>(tl, chunks, sci, sval, perm, xp, yp, alpha, beta);
+      continue;
+    }
+    const BatchProblem pb = probs[tl.prob];
     const int G = shape.lanes[pb.kclass];
     const int cpl = shape.cpl[pb.kclass];
     if (cpl == 1)
@@ -166,14 +219,15 @@ hipError_t batch_launch(const BatchLaunch &L, hipStream_t s) {
     using X = std::remove_pointer_t<decltype(xtag)>;
     const auto *xs = static_cast<const X *const *>(L.xs);
     auto *ys = static_cast<X *const *>(L.ys);
+    const auto *sv = static_cast<const V *>(L.svals);
     if (L.beta != 0.0)
       k_batch<V, X, true><<<L.grid, kBlock, 0, s>>>(
           L.ntiles, L.tiles, L.probs, L.rp, L.ci, static_cast<const V *>(L.vals), xs, ys,
-          acc_t<X>(L.alpha), acc_t<X>(L.beta), L.shape);
+          acc_t<X>(L.alpha), acc_t<X>(L.beta), L.shape, L.sell_C, L.chunks, L.sci, sv, L.perm);
     else
       k_batch<V, X, false><<<L.grid, kBlock, 0, s>>>(
           L.ntiles, L.tiles, L.probs, L.rp, L.ci, static_cast<const V *>(L.vals), xs, ys,
-          acc_t<X>(L.alpha), acc_t<X>(0), L.shape);
+          acc_t<X>(L.alpha), acc_t<X>(0), L.shape, L.sell_C, L.chunks, L.sci, sv, L.perm);
     return hipGetLastError();
   });
 }

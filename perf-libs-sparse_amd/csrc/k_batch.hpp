@@ -27,8 +27,20 @@ struct BatchProblem {
   int32_t m, k, kclass, pad;
 };
 
+// kind 0: rows [r0, r1) of problem `prob` (CSR); kind 1: SELL-C-sigma chunks
+// [r0, r1) of the batch's chunk table, all of problem `prob` (k = 1 only).
 struct BatchTile {
-  int32_t prob, r0, r1, pad;  // rows [r0, r1) of problem `prob`
+  int32_t prob, r0, r1, kind;
+};
+
+// A SELL-C-sigma chunk of a k = 1 problem (src/sell_c_sigma.cpp:145-336 layout
+// per problem: rows sorted by length inside sigma-row windows, C rows per
+// chunk): `width` column-major slices of C entries from `off` in the batch's
+// SELL column / value arrays (zero-valued padding), the row of slot r at
+// perm[slot0 + r] (-1 for the empty slots of a problem's last chunk).
+struct BatchSellChunk {
+  int64_t off;
+  int32_t width, slot0;
 };
 
 // Lanes per row (a power of two, at least k / cpl) and columns per lane
@@ -50,6 +62,11 @@ struct BatchLaunch {
   const void *ys = nullptr;  // device array of nprob Y pointers
   double alpha = 1.0, beta = 0.0;
   BatchShape shape{};
+  // SELL-C-sigma tiles (sell_C 32 or 64; 0: the batch has none)
+  int32_t sell_C = 0;
+  const BatchSellChunk *chunks = nullptr;
+  const int32_t *sci = nullptr, *perm = nullptr;
+  const void *svals = nullptr;
 };
 
 hipError_t batch_launch(const BatchLaunch &L, hipStream_t s);

@@ -137,6 +137,8 @@ _SIGS = {
     "perflibs_amd_batch_optimize": (_I, [_P, C.c_int]),
     "perflibs_amd_batch_set_shape": (_I, [_P, _I, _P, _P]),
     "perflibs_amd_batch_exec": (_I, [_P, _D, _D]),
+    "perflibs_amd_batch_set_sell": (_I, [_P, _I, _I]),
+    "perflibs_amd_batch_sell_info": (_I, [_P, _P, _P, _P]),
     "perflibs_amd_batch_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "perflibs_amd_probe_panel_gathers": (_I, [_P, _P, _P]),
     "perflibs_amd_probe_random_gathers": (_I, [_P, C.c_int64, C.c_int64, _P]),
@@ -586,6 +588,15 @@ class Batch:
 
     def exec(self, alpha=1.0, beta=0.0):
         check("batch_exec", lib().perflibs_amd_batch_exec(self.h, alpha, beta))
+
+    def set_sell(self, Cc: int, sigma: int):
+        """Run the k = 1 problems as SELL-C-sigma tiles (Cc = 0: CSR row tiles)."""
+        check("batch_set_sell", lib().perflibs_amd_batch_set_sell(self.h, Cc, sigma))
+
+    def sell_info(self) -> dict:
+        cc, sg, pad = C.c_int32(), C.c_int32(), C.c_int64()
+        check("batch_sell_info", lib().perflibs_amd_batch_sell_info(self.h, C.byref(cc), C.byref(sg), C.byref(pad)))
+        return {"C": cc.value, "sigma": sg.value, "padded": pad.value}
 
     def info(self) -> dict:
         nt = C.c_int64()

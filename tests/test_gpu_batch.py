@@ -218,3 +218,83 @@ def test_batch_multi_column_pieces_and_unaligned_operands(pls, orc, gen, dtype):
             torch.cuda.synchronize()
             for i in range(len(probs)):
                 assert rel_err(Y[i].cpu().numpy(), ref[i]) <= tol, (cpl, i)
+
+
+@pytest.mark.parametrize("Cc,sigma", [(32, 0), (32, 256), (64, 64), (64, 1024)])
+@pytest.mark.parametrize("dtype,precision", [(np.float64, 0), (np.float32, 0), (np.float32, 2)])
+def test_batch_sell_tiles(pls, orc, gen, Cc, sigma, dtype, precision):
+    """k = 1 problems as SELL-C-sigma chunk tiles beside CSR row tiles of the
+    k > 1 problems: every problem against the oracle (FP64 1e-12; FP32 / BF16
+    storage within 2e-6 of the exact product of the stored values), for
+    alpha / beta with and without reading Y, and a return to CSR tiles."""
+    import torch
+
+    probs, B = make(pls, gen, 30, 51 + Cc + sigma, rows=(40, 5000), ks=(1, 1, 4, 16), dtype=dtype,
+                    precision=precision, empty=False)
+    m0 = 9  # a k = 1 problem with empty rows and a partial last chunk
+    rp = np.array([0, 0, 3, 3, 4, 9, 9, 9, 10, 10], np.int32)
+    ci = np.array([0, 4, 8, 2, 1, 2, 3, 5, 7, 6], np.int32)
+    probs.append((m0, rp, ci, np.arange(1.0, 11.0), 1))
+    B = pls.Batch([pls.SpMat.csr(m, m, r, c, v.astype(dtype)) for (m, r, c, v, k) in probs],
+                  [p[4] for p in probs], precision)
+    X, Y, Y0, _ = operands(probs, 52, dtype)
+    B.set_operands([x.data_ptr() for x in X], [y.data_ptr() for y in Y])
+    B.set_sell(Cc, sigma)
+    info = B.sell_info()
+    assert (info["C"], info["sigma"]) == (Cc, sigma)
+    if dtype == np.float64:
+        stored = [v.astype(np.float64) for (_, _, _, v, _) in probs]
+    elif precision == 2:
+        stored = [to_bf16(v.astype(np.float32)).astype(np.float64) for (_, _, _, v, _) in probs]
+    else:
+        stored = [v.astype(np.float32).astype(np.float64) for (_, _, _, v, _) in probs]
+    tol = TOL64 if dtype == np.float64 else 2e-6
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        for i, y in enumerate(Y):
+            y.copy_(torch.from_numpy(Y0[i]))
+        B.exec(alpha, beta)
+        torch.cuda.synchronize()
+        ref = oracle_of(orc, probs, X, alpha, beta, Y0, vals=stored)
+        for i, (m, _, _, _, k) in enumerate(probs):
+            assert rel_err(Y[i].cpu().numpy(), ref[i]) <= tol, (Cc, sigma, alpha, i, m, k)
+    B.set_sell(0, 0)
+    assert B.sell_info()["C"] == 0
+    for y in Y:
+        y.fill_(float("nan"))
+    B.exec(1.0, 0.0)
+    torch.cuda.synchronize()
+    ref = oracle_of(orc, probs, X, 1.0, 0.0, Y0, vals=stored)
+    for i in range(len(probs)):
+        assert rel_err(Y[i].cpu().numpy(), ref[i]) <= tol
+
+
+def test_batch_sell_padding_matches_layout_rule(pls, gen):
+    """Padded entries of the batch's SELL-C-sigma layout = sum over its k = 1
+    problems of sum_c width_c * C, widths the longest row of each chunk after
+    sorting rows by length inside sigma windows (checked here from the row
+    lengths alone), and sorting windows cut the padding of banded rows."""
+    probs = gen.banded_batch(12, 61, rows=(100, 3000), ks=(1,))
+    B = pls.Batch([pls.SpMat.csr(m, m, r, c, v) for (m, r, c, v, k) in probs], [1] * len(probs))
+
+    def padded(Cc, sigma):
+        tot = 0
+        for (m, r, c, v, k) in probs:
+            lens = np.diff(r).astype(np.int64)
+            win = sigma if sigma else Cc
+            nwin = -(-m // win)
+            lp = np.zeros(nwin * win, np.int64)
+            lp[:m] = lens
+            if sigma:
+                lp = -np.sort(-lp.reshape(nwin, win), axis=1).reshape(-1)
+            nC = -(-m // Cc)
+            tot += int(lp[: nC * Cc].reshape(nC, Cc).max(axis=1).sum()) * Cc
+        return tot
+
+    for Cc, sigma in ((32, 0), (32, 256), (64, 0), (64, 1024)):
+        B.set_sell(Cc, sigma)
+        assert B.sell_info()["padded"] == padded(Cc, sigma), (Cc, sigma)
+    assert padded(64, 1024) < 0.7 * padded(64, 0)
+    with pytest.raises(pls.PerflibsError):
+        B.set_sell(48, 96)
+    with pytest.raises(pls.PerflibsError):
+        B.set_sell(32, 48)
