@@ -126,8 +126,12 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
 // column / value slices of a chunk are contiguous across its lanes, so each
 // step of the row loop is one coalesced load per array; padding slots hold
 // value 0 and column 0 of the problem.  Four non-zeros in flight per lane.
+// Kept out of line: inlined into k_batch it changed the register allocation of
+// the CSR tile loop and made the CSR tiles 1.6x slower (FP32 batch 4.77 ->
+// 7.52 ms, FP64 7.89 -> 12.71 ms at the same shape); as a call it costs the
+// CSR tiles nothing.
 template <int C, class V, class X, bool BETA>
-__device__ __forceinline__ void batch_sell_tile(const BatchTile &tl,
+__device__ __noinline__ void batch_sell_tile(const BatchTile &tl,
                                                 const BatchSellChunk *__restrict__ chunks,
                                                 const int32_t *__restrict__ sci,
                                                 const V *__restrict__ sval,

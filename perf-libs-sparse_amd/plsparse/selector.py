@@ -143,10 +143,10 @@ def collect(count: int, seed: int, out: Path, log=print, max_nnz: int = MAX_NNZ)
                      max_nnz=np.full(len(seeds), max_nnz, np.int64))
 
 
-def merge(parts, out: Path, legacy=()):
-    """Concatenate collect outputs.  legacy: (file, run seed, max_nnz) for files
-    written before seeds were recorded, whose matrix i has seed
-    matrix_seed(run seed, i)."""
+def merge(parts, out: Path, legacy=(), limit: int | None = None):
+    """Concatenate collect outputs (the first `limit` matrices when given).
+    legacy: (file, run seed, max_nnz) for files written before seeds were
+    recorded, whose matrix i has seed matrix_seed(run seed, i)."""
     legacy = {str(f): (s, mx) for f, s, mx in legacy}
     cols = {k: [] for k in ("features", "times", "heuristic", "kinds", "sizes", "seeds", "max_nnz")}
     for f in parts:
@@ -161,7 +161,7 @@ def merge(parts, out: Path, legacy=()):
             rs, mx = legacy[str(f)]
             cols["seeds"].append(np.array([matrix_seed(rs, i) for i in range(n)], np.int64))
             cols["max_nnz"].append(np.full(n, mx, np.int64))
-    np.savez(out, **{k: np.concatenate(v) for k, v in cols.items()})
+    np.savez(out, **{k: np.concatenate(v)[:limit] for k, v in cols.items()})
 
 
 def targets(times: np.ndarray):
@@ -304,6 +304,7 @@ def main(argv=None) -> int:
     mg = sub.add_parser("merge")
     mg.add_argument("--out", required=True)
     mg.add_argument("parts", nargs="+")
+    mg.add_argument("--limit", type=int, default=None, help="keep the first LIMIT matrices")
     mg.add_argument("--legacy", action="append", default=[],
                     help="FILE:RUN_SEED:MAX_NNZ for a part written without per-matrix seeds")
     t = sub.add_parser("train")
@@ -330,7 +331,7 @@ def main(argv=None) -> int:
         for item in args.legacy:
             f, rs, mx = item.rsplit(":", 2)
             legacy.append((f, int(rs), int(mx)))
-        merge(args.parts, Path(args.out), legacy)
+        merge(args.parts, Path(args.out), legacy, args.limit)
     elif args.cmd == "train":
         d = np.load(args.data)
         feats, times = d["features"], d["times"]
