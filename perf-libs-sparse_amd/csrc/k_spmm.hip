@@ -393,79 +393,108 @@ __global__ __launch_bounds__(kBlock) void k_csr_few_rhs(
   }
 }
 
-// Native BSR x dense (b = 4 or 8, row-major B and C): a wave holds 64 / (b * G)
-// block rows and a column window of each.  Lane (i, cl) of a block row owns row
-// i and CPL adjacent columns c0 + cl * CPL + t (G column lanes per row); for
-// each block the lanes of its block row load the b x b values with one
-// coalesced load into the wave's LDS slice and each lane reads its row's b
-// values back, and the b rows of B the block names are read once per block
-// row (the b lanes that share a column read the same address), so A streams
-// at its 8 bytes per value and every B row is reused by the block's b rows.
-// A stored zero adds nothing (0 * Inf stays out of the sum), matching the
-// product of the zero-dropped CSR expansion the reference computes
-// (src/matmul.cpp:619-649, src/compressed_sparse_rows.cpp:966-1056).
-template <int BS, int G, int CPL, bool VEC, class V, class X, bool BETA, bool COLMAJ>
+// Native BSR x dense (b = 4 or 8, row-major B and C) with a register tile per
+// lane: NGC lanes span a block row's column window (CPL adjacent columns each)
+// and every lane accumulates all BS rows of the block row for its columns, so
+// each B value reaches one lane only.  (With one row per lane the BS row lanes
+// of a block each receive the same B pieces: on 8 x 8 blocks at k = 32 that was
+// 16 KB of L1 -> register traffic per block, the texture path stayed busy 84%
+// of the kernel, and the product took 4.0 ms instead of 2.5.)  A
+// block's values pass through the wave's LDS slice once, stored by columns;
+// the lanes of the block row read column jj (BS values, one broadcast read) as
+// the multipliers of B row jj.  The next block's column and values are loaded
+// while the current block is multiplied.  A stored zero adds nothing, as in the
+// zero-dropped CSR product of the reference (src/matmul.cpp:619-649).
+template <int BS, int NGC, int CPL, bool VEC, class V, class X, bool BETA, bool COLMAJ>
 __global__ __launch_bounds__(kBlock) void k_bsr_spmm(
     int32_t mb, int32_t npass, int32_t k, const int32_t *__restrict__ rp,
     const int32_t *__restrict__ ci, const V *__restrict__ val, const X *__restrict__ Bm,
     int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha, acc_t<X> beta) {
   using A = acc_t<X>;
-  constexpr int BB = BS * BS;      // values per block
-  constexpr int GL = BS * G;       // lanes of one block row
-  constexpr int RPW = 64 / GL;     // block rows per wave
-  static_assert(GL >= BB && RPW >= 1, "a block row's lanes load its blocks in one pass");
-  __shared__ A s_blk[kBlock / 64][64];
+  constexpr int BB = BS * BS;                  // values per block
+  constexpr int RPW = 64 / NGC;                // block rows per wave
+  constexpr int VPL = (BB + NGC - 1) / NGC;    // block values each lane of a block row loads
+  __shared__ __align__(16) A s_blk[kBlock / 64][RPW][BB];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int sub = lane / GL, i = (lane % GL) / G, cl = lane % G;
+  const int slot = lane / NGC, g = lane % NGC;
   const int64_t w = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * (kBlock / 64) + wave;
-  const int64_t unit = w * RPW + sub;  // (block row, column window) of this lane
+  const int64_t unit = w * RPW + slot;  // (block row, column window) of this lane
   const bool live = unit < int64_t(mb) * npass;
   const int32_t br = live ? int32_t(unit / npass) : 0;
-  const int32_t c0 = live ? int32_t(unit % npass) * (G * CPL) + cl * CPL : k;
+  const int32_t c0 = live ? int32_t(unit % npass) * (NGC * CPL) + g * CPL : k;
   const int32_t p0 = live ? rp[br] : 0, p1 = live ? rp[br + 1] : 0;
   // every lane of the wave runs the longest block row's count of steps
   int32_t steps = p1 - p0;
 #pragma unroll
-  for (int o = GL; o < 64; o <<= 1) steps = max(steps, __shfl_xor(steps, o));
-  A acc[CPL];
+  for (int o = NGC; o < 64; o <<= 1) steps = max(steps, __shfl_xor(steps, o));
+  A acc[BS][CPL];
 #pragma unroll
-  for (int t = 0; t < CPL; ++t) acc[t] = A(0);
-  A *blk = s_blk[wave] + sub * GL;
-  for (int32_t q = 0; q < steps; ++q) {
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) acc[i][t] = A(0);
+  A *blk = s_blk[wave][slot];
+  auto fetch = [&](int32_t q, int32_t &bc, A(&v)[VPL]) {
     const bool on = p0 + q < p1;
     const int32_t p = on ? p0 + q : p0;
-    const int32_t bc = on ? ci[p] : 0;
-    blk[lane % GL] = on ? A(ldv(val, int64_t(p) * BB + ((lane % GL) % BB))) : A(0);
-    __builtin_amdgcn_wave_barrier();
-    A a[BS];
+    bc = on ? ci[p] : 0;
 #pragma unroll
-    for (int jj = 0; jj < BS; ++jj) a[jj] = blk[COLMAJ ? jj * BS + i : i * BS + jj];
+    for (int u = 0; u < VPL; ++u) {
+      const int e = g + u * NGC;
+      v[u] = on && e < BB ? A(ldv(val, int64_t(p) * BB + e)) : A(0);
+    }
+  };
+  int32_t bc_n = 0;
+  A v_n[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) v_n[u] = A(0);
+  if (steps > 0) fetch(0, bc_n, v_n);
+  for (int32_t q = 0; q < steps; ++q) {
+    const bool on = p0 + q < p1;
+    const int32_t bc = bc_n;
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+      const int e = g + u * NGC;  // storage position; the slice holds column jj at jj * BS
+      if (e < BB) blk[COLMAJ ? e : (e % BS) * BS + e / BS] = v_n[u];
+    }
+    if (q + 1 < steps) fetch(q + 1, bc_n, v_n);
+    A b[BS][CPL];
+#pragma unroll
+    for (int jj = 0; jj < BS; ++jj) {
+      const X *bp = Bm + (int64_t(bc) * BS + jj) * ldb + c0;
+      if constexpr (VEC) {
+        if (on) Piece<CPL, X>::load(bp, b[jj]);
+        else
+#pragma unroll
+          for (int t = 0; t < CPL; ++t) b[jj][t] = A(0);
+      } else {
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) b[jj][t] = on && c0 + t < k ? A(bp[t]) : A(0);
+      }
+    }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int jj = 0; jj < BS; ++jj) {
-      const X *b = Bm + (int64_t(bc) * BS + jj) * ldb + c0;
-      A bv[CPL];
-      if constexpr (VEC) {
-        if (on) Piece<CPL, X>::load(b, bv);
-        else
+      A a[BS];
 #pragma unroll
-          for (int t = 0; t < CPL; ++t) bv[t] = A(0);
-      } else {
+      for (int i = 0; i < BS; ++i) a[i] = blk[jj * BS + i];
 #pragma unroll
-        for (int t = 0; t < CPL; ++t) bv[t] = on && c0 + t < k ? A(b[t]) : A(0);
-      }
+      for (int i = 0; i < BS; ++i)
 #pragma unroll
-      for (int t = 0; t < CPL; ++t) acc[t] += a[jj] != A(0) ? a[jj] * bv[t] : A(0);
+        for (int t = 0; t < CPL; ++t) acc[i][t] += a[i] != A(0) ? a[i] * b[jj][t] : A(0);
     }
+    __builtin_amdgcn_wave_barrier();
   }
   if (!live) return;
-  X *c = Cm + (int64_t(br) * BS + i) * ldc + c0;
 #pragma unroll
-  for (int t = 0; t < CPL; ++t) {
-    if (c0 + t < k) {
-      A r = alpha * acc[t];
-      if (BETA) r += beta * A(c[t]);
-      c[t] = X(r);
+  for (int i = 0; i < BS; ++i) {
+    X *c = Cm + (int64_t(br) * BS + i) * ldc + c0;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+      if (c0 + t < k) {
+        A r = alpha * acc[i][t];
+        if (BETA) r += beta * A(c[t]);
+        c[t] = X(r);
+      }
     }
   }
 }
@@ -670,7 +699,7 @@ template <int BS, int G, int CPL, class V, class X>
 void launch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
                      double beta, void *C, int64_t ldc, hipStream_t s) {
   constexpr int WCOLS = G * CPL;          // columns of one block row's window
-  constexpr int RPW = 64 / (BS * G);      // block rows per wave
+  constexpr int RPW = 64 / G;             // block rows per wave
   const int32_t npass = (k + WCOLS - 1) / WCOLS;
   const int64_t units = int64_t(A.mb) * npass;
   const int64_t waves = (units + RPW - 1) / RPW;
@@ -695,22 +724,21 @@ void launch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *
   if (vec) beta != 0.0 ? by_layout(T_{}, T_{}) : by_layout(T_{}, F_{});
   else beta != 0.0 ? by_layout(F_{}, T_{}) : by_layout(F_{}, F_{});
 }
-// Window shapes from k: 8 x 8 blocks keep 8 column lanes per row (one block
-// row per wave) with 1, 2 or 4 columns per lane; 4 x 4 blocks take 4, 8 or 16
-// column lanes (4, 2 or 1 block rows per wave), then 2 or 4 columns per lane.
+// Window shapes from k (column lanes x columns per lane): 8 x 8 blocks 8 x 1,
+// 8 x 2, then 16 x 2 (a 32-column window, four block rows per wave, 108-118
+// VGPRs); 4 x 4 blocks 4 x 1, 8 x 1, 8 x 2, then 8 x 4.
 template <int BS, class V, class X>
 void dispatch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *B, int64_t ldb,
                        double beta, void *C, int64_t ldc, hipStream_t s) {
   if constexpr (BS == 8) {
     if (k <= 8) launch_bsr_spmm<8, 8, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
     else if (k <= 16) launch_bsr_spmm<8, 8, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
-    else launch_bsr_spmm<8, 8, 4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else launch_bsr_spmm<8, 16, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
   } else {
     if (k <= 4) launch_bsr_spmm<4, 4, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
     else if (k <= 8) launch_bsr_spmm<4, 8, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
-    else if (k <= 16) launch_bsr_spmm<4, 16, 1, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
-    else if (k <= 32) launch_bsr_spmm<4, 16, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
-    else launch_bsr_spmm<4, 16, 4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else if (k <= 16) launch_bsr_spmm<4, 8, 2, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
+    else launch_bsr_spmm<4, 8, 4, V, X>(A, k, alpha, B, ldb, beta, C, ldc, s);
   }
 }
 }  // namespace
