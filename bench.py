@@ -307,7 +307,8 @@ def spmm_lines(steps: int, warmup: int, with_cpu: bool) -> dict:
     """CSR x dense SpMM: the 27-point stencil on 128^3 (n = 2,097,152) times a
     row-major B of k in {8, 32, 128} columns, FP64 and FP32, and the BSR
     operator of the same shape (27-point block pattern of dense N(0,1) 4x4 or
-    8x8 blocks) in FP64.  Device-resident B and C, beta = 0; GB/s counts
+    8x8 blocks) in FP64, with the SpMV of that BSR operator on its native block
+    kernel beside it.  Device-resident B and C (x and y), beta = 0; GB/s counts
     values + indices + B + C once (BSR: block values + block indices)."""
     import numpy as np
     import torch
@@ -351,6 +352,29 @@ def spmm_lines(steps: int, warmup: int, with_cpu: bool) -> dict:
     for b, grid in ((4, (128, 64, 64)), (8, (64, 64, 64))):
         brp, bci, bv = W.block_stencil(*grid, b, 0x5EED0003)
         nnzb = int(brp[-1])
+        # SpMV on the native BSR plan (a handle of its own: the SpMM below turns
+        # its handle into CSR, as the reference's SpMM does)
+        S = P.SpMat.bsr(P.ROW_MAJOR, n, n, b, brp, bci, bv, flags=P.CREATE_NOCOPY)
+        xd = torch.rand(n, dtype=torch.float64, device="cuda") * 2 - 1
+        yd = torch.empty(n, dtype=torch.float64, device="cuda")
+        for _ in range(warmup):
+            S.spmv_async(xd.data_ptr(), yd.data_ptr())
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            S.spmv_async(xd.data_ptr(), yd.data_ptr())
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        nbytes = nnzb * b * b * 8 + nnzb * 4 + (n // b + 1) * 4 + n * 8 * 2
+        lines[f"bsr{b}_fp64_spmv"] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
+                                      "hbm_fraction": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                      "GFLOP/s": round(2 * nnzb * b * b / ms / 1e6, 1),
+                                      "algorithmic_bytes": nbytes, "variant": S.variant()}
+        log(f"bsr{b}_fp64_spmv: {ms:.3f} ms {nbytes / ms / 1e6:.1f} GB/s")
+        S.destroy()
+        del xd, yd
         A = P.SpMat.bsr(P.ROW_MAJOR, n, n, b, brp, bci, bv, flags=P.CREATE_NOCOPY)
         for k in (8, 32):
             nbytes = nnzb * b * b * 8 + nnzb * 4 + (n // b + 1) * 4 + n * k * 8 * 2
