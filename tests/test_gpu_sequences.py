@@ -2,8 +2,9 @@
 checked against a dense FP64 model of the object.
 
 Every sequence starts from a CSR or BSR object (rectangular shapes, empty rows)
-and applies a random mix of hints, optimize() under each selection mode or a
-forced variant, SpMV and SpMM (NOTRANS and TRANS, alpha / beta with and without
+and applies a random mix of hints, optimize() under each selection mode, a
+forced variant, SELL-C-sigma shape or column-panel shape, BF16 value storage
+switched on and off (FP32 objects), SpMV and SpMM (NOTRANS and TRANS, alpha / beta with and without
 reading y), value updates, in-place scaling and transposition, norms and
 export.  The model is numpy's dense FP64 matrix of the same stored values; results must
 agree to a normwise relative error of 1e-12 (FP64 objects) or 2e-6 (FP32
@@ -20,6 +21,13 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+
+
+def to_bf16(a):
+    """Round to the nearest BF16 (ties to even), returned as FP64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
 def make_object(pls, rng, dt):
@@ -59,10 +67,13 @@ def run_sequence(pls, seed, steps=14, dt=np.float64):
     A, D, kind = make_object(pls, rng, dt)
     log = [f"seed {seed} {kind} {D.shape}"]
     done = []
+    bf16 = False  # BF16 value storage on the device (FP32 objects)
+    ops = ["hint", "optimize", "spmv", "spmv", "spmm", "update", "scale", "transpose", "norm",
+           "export", "layout"] + (["precision"] if dt == np.float32 else [])
     for _ in range(steps):
-        op = rng.choice(["hint", "optimize", "spmv", "spmv", "spmm", "update", "scale",
-                         "transpose", "norm", "export"])
+        op = rng.choice(ops)
         m, n = D.shape
+        P = to_bf16(D) if bf16 else D  # the values the products use
         log.append(op)
         if op == "hint":
             A.hint(pls.HINT_SPMV_OPERATION, int(rng.choice([pls.NOTRANS, pls.TRANS])))
@@ -93,7 +104,7 @@ def run_sequence(pls, seed, steps=14, dt=np.float64):
                 y = yd.cpu().numpy()
             else:
                 A.spmv(x, y, alpha, beta, trans)
-            opD = D if trans == pls.NOTRANS else D.T
+            opD = P if trans == pls.NOTRANS else P.T
             ref = alpha * (opD @ x.astype(np.float64)) + (beta * y0.astype(np.float64) if beta != 0.0 else 0.0)
             assert rel_err(y, ref) <= tol, log
         elif op == "spmm":
@@ -107,7 +118,7 @@ def run_sequence(pls, seed, steps=14, dt=np.float64):
             Bm = pls.SpMat.dense(pls.ROW_MAJOR, inner, k, k, Bv.reshape(-1))
             Cm = pls.SpMat.dense(pls.ROW_MAJOR, rows, k, k, C0.reshape(-1))
             pls.spmm_exec(transA, pls.NOTRANS, alpha, A, Bm, beta, Cm)
-            opD = D if transA == pls.NOTRANS else D.T
+            opD = P if transA == pls.NOTRANS else P.T
             ref = alpha * opD @ Bv.astype(np.float64) + beta * C0.astype(np.float64)
             assert rel_err(Cm.export_dense(pls.ROW_MAJOR), ref) <= tol, log
         elif op == "update":
@@ -140,6 +151,19 @@ def run_sequence(pls, seed, steps=14, dt=np.float64):
                     np.abs(D).sum(axis=1).max() if D.size else 0.0, rel=tol, abs=1e-300), log
             else:
                 assert st == 1, log
+        elif op == "layout":  # a forced SELL-C-sigma shape or column-panel shape, then optimize
+            if rng.random() < 0.5:
+                Cc = int(rng.choice([1, 5, 16, 32, 64]))
+                A.force_sell(Cc, int(rng.choice([0, Cc, 4 * Cc])))
+            else:
+                A.force_panel(int(rng.choice([0, 6, 9])), int(rng.choice([0, 2])))
+                A.force_variant(12)
+            st = pls.lib().perflibs_spmv_optimize(A.h)
+            assert st in (0, 1, 2), log
+            A.force_variant(-1)
+        elif op == "precision":
+            bf16 = not bf16
+            A.set_precision(pls.VALUES_BF16 if bf16 else pls.VALUES_NATIVE)
         elif op == "export":
             mm, nn, rp, ci, v = A.export_csr(0)
             E = np.zeros((mm, nn))
@@ -158,5 +182,6 @@ def test_random_call_sequences(pls, block, dtype):
         for op in run_sequence(pls, 0x5E0000 + seed, steps=16, dt=dtype):
             counts[op] = counts.get(op, 0) + 1
     # every kind of call ran in this block of sequences
-    assert set(counts) == {"hint", "optimize", "spmv", "spmm", "update", "scale", "transpose",
-                           "norm", "export"}, counts
+    expect = {"hint", "optimize", "spmv", "spmm", "update", "scale", "transpose", "norm",
+              "export", "layout"} | ({"precision"} if dtype == np.float32 else set())
+    assert set(counts) == expect, counts
