@@ -348,6 +348,7 @@ def spmm_lines(steps: int, warmup: int, with_cpu: bool) -> dict:
             nbytes = nnz * (sv + 4) + (n + 1) * 4 + n * k * sv * 2
             run(f"csr_{prec}_k{k}", A, k, tdt, nbytes)
             lines[f"csr_{prec}_k{k}"]["GFLOP/s"] = round(2 * nnz * k / lines[f"csr_{prec}_k{k}"]["ms"] / 1e6, 1)
+            lines[f"csr_{prec}_k{k}"]["row_blocks"] = "clustered" if A.spmm_rows_clustered() else "consecutive"
         A.destroy()
     for b, grid in ((4, (128, 64, 64)), (8, (64, 64, 64))):
         brp, bci, bv = W.block_stencil(*grid, b, 0x5EED0003)

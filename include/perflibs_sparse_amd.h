@@ -122,6 +122,15 @@ perflibs_status_t perflibs_amd_force_panel(perflibs_spmat_t A, perflibs_int_t pa
 perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blocks_state,
                                          perflibs_int_t *nblocks, perflibs_int_t *list_max,
                                          int64_t *list_total);
+/* How the SpMM plan groups rows into those blocks: 0 (default) keeps the cut
+ * with the shorter column lists of two, consecutive rows or rows clustered by
+ * shared columns (a block grows by the row that adds the fewest unlisted
+ * columns); 1 cuts consecutive rows only; 2 takes the clustered cut whenever
+ * it completes.  Takes effect at the next SpMM with k >= 8. */
+perflibs_status_t perflibs_amd_set_spmm_rows(perflibs_spmat_t A, int rows);
+/* *clustered = 1 when the built blocks hold clustered rows, 0 for consecutive
+ * rows; status 1 before the blocks are built. */
+perflibs_status_t perflibs_amd_spmm_rows_clustered(perflibs_spmat_t A, int *clustered);
 /* Shape of a column-panel plan (for tests and tuning). */
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,

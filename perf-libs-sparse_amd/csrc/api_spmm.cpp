@@ -159,7 +159,7 @@ perflibs_status_t csr_plan_for(perflibs_spmat_top_t *A, int op, int64_t k) {
   if (k >= 8 && A->plan.csr->lb_state == 0) {
     if (!o.prp && !pls::make_operand(A, op, o))
       return pls::exec_fail(A, "matrix format has no GPU SpMM path");
-    const hipError_t e = pls::build_spmm_blocks(o, *A->plan.csr);
+    const hipError_t e = pls::build_spmm_blocks(o, *A->plan.csr, A->spmm_rows);
     if (e != hipSuccess) return pls::hip_fail(A, e, "SpMM block upload");
   }
   return PERFLIBS_STATUS_SUCCESS;

@@ -311,6 +311,20 @@ perflibs_status_t perflibs_amd_spmm_info(perflibs_spmat_t A, perflibs_int_t *blo
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_set_spmm_rows(perflibs_spmat_t A, int rows) {
+  if (!A || rows < 0 || rows > 2) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  A->spmm_rows = rows;
+  if (A->plan.csr) A->plan.csr->lb_state = 0;  // cut again at the next SpMM with k >= 8
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
+perflibs_status_t perflibs_amd_spmm_rows_clustered(perflibs_spmat_t A, int *clustered) {
+  if (!A || !clustered || !A->plan.csr || A->plan.csr->lb_state == 0)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  *clustered = A->plan.csr->lb_clustered ? 1 : 0;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t perflibs_amd_probe_panel_gathers(perflibs_spmat_t A, const void *x_dev,
                                                   void *out_dev) {
   if (!A || !x_dev || !out_dev || !A->plan.panel) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;

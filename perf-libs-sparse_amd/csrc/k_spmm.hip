@@ -175,7 +175,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 
 // Column-compressed row blocks (dispatched for k >= 8 when the plan's block
 // lists are short enough, see build_spmm_blocks): one workgroup per block of
-// consecutive rows.  The block's alpha-scaled values and one-byte list
+// rows, consecutive ones or (brow != nullptr) the rows brow lists for the
+// block, whose non-zeros the plan keeps in block order (prp, lval, lidx: one
+// contiguous range per block, like consecutive rows).  The block's
+// alpha-scaled values and one-byte list
 // positions are staged in LDS once; then, KC columns at a time, every B row the
 // block needs is loaded into LDS once (coalesced, KC contiguous elements per
 // row) and each row of C is accumulated from LDS by a group of KC / CPL lanes
@@ -188,10 +191,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 template <int KC, int UMAX, int CPL, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
-    const int32_t *__restrict__ ucol, const uint8_t *__restrict__ lidx, int32_t k,
-    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const V *__restrict__ val,
-    const X *__restrict__ B, int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha,
-    acc_t<X> beta) {
+    const int32_t *__restrict__ ucol, const uint8_t *__restrict__ lidx,
+    const int32_t *__restrict__ brow, const int32_t *__restrict__ prp,
+    const V *__restrict__ lval, int32_t k, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const V *__restrict__ val, const X *__restrict__ B,
+    int64_t ldb, X *__restrict__ Cm, int64_t ldc, acc_t<X> alpha, acc_t<X> beta) {
   using A = acc_t<X>;
   constexpr int NG = kBlock / KC;  // lane groups (rows in flight) per workgroup
   // B rows in flight per thread while staging: up to 8, at most the list rows
@@ -202,25 +206,31 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   __shared__ uint8_t s_l[pls::lb_nnz_per_ucol(UMAX) * UMAX];
   __shared__ int32_t s_rp[pls::kLbRows + 1];
   __shared__ int32_t s_uc[UMAX];
+  __shared__ int32_t s_row[pls::kLbRows];  // clustered blocks: the rows' indices
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x, lane = tid % KC, grp = tid / KC;
-  const int32_t r0 = row0s[blk], r1 = row0s[blk + 1], nrows = r1 - r0;
+  // r0: first entry of the row order; the block's non-zeros are [e0, e0 + nnz)
+  // of the block-order arrays (the CSR arrays for consecutive rows)
+  const int32_t r0 = row0s[blk], nrows = row0s[blk + 1] - r0;
   const int32_t u0 = uoff[blk], nu = uoff[blk + 1] - u0;
-  const int32_t e0 = rp[r0], nnz = rp[r1] - e0;
+  const int32_t *brp = brow ? prp : rp;
+  const V *bval = brow ? lval : val;
+  const int32_t e0 = brp[r0], nnz = brp[r0 + nrows] - e0;
 
   if (nu == 0 && nnz > 0) {  // one long row without a list: B read directly
+    const int32_t row = brow ? brow[r0] : r0, f0 = rp[row];  // its CSR non-zeros
     for (int32_t c0 = 0; c0 < k; c0 += KC) {
       const int32_t col = c0 + lane;
       A acc = 0;
       if (col < k)
         for (int32_t p = grp; p < nnz; p += NG)
-          acc += alpha * A(ldv(val, e0 + p)) * A(B[int64_t(ldi(ci, e0 + p)) * ldb + col]);
+          acc += alpha * A(ldv(val, f0 + p)) * A(B[int64_t(ldi(ci, f0 + p)) * ldb + col]);
       sB[grp * KC + lane] = acc;
       __syncthreads();
       if (grp == 0 && col < k) {
         A r = 0;
         for (int g = 0; g < NG; ++g) r += sB[g * KC + lane];
-        X *c = Cm + int64_t(r0) * ldc + col;
+        X *c = Cm + int64_t(row) * ldc + col;
         if (BETA) r += beta * A(*c);
         *c = X(r);
       }
@@ -230,10 +240,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   }
 
   for (int32_t q = tid; q < nnz; q += kBlock) {
-    s_a[q] = alpha * A(ldv(val, e0 + q));
+    s_a[q] = alpha * A(ldv(bval, e0 + q));
     s_l[q] = lidx[e0 + q];
   }
-  for (int32_t q = tid; q <= nrows; q += kBlock) s_rp[q] = rp[r0 + q] - e0;
+  for (int32_t q = tid; q <= nrows; q += kBlock) {
+    s_rp[q] = brp[r0 + q] - e0;
+    if (brow && q < nrows) s_row[q] = brow[r0 + q];
+  }
   for (int32_t q = tid; q < nu; q += kBlock) s_uc[q] = ucol[u0 + q];
   __syncthreads();
   // compute mapping: G = KC / CPL lanes per row group, CPL adjacent columns
@@ -294,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
       for (int t = 0; t < CPL; ++t)
         for (int o = G; o < G * S; o <<= 1) acc[t] += __shfl_xor(acc[t], o);
       if (sub == 0) {
-        X *c = Cm + int64_t(r0 + rr) * ldc;
+        X *c = Cm + int64_t(brow ? s_row[rr] : r0 + rr) * ldc;
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
           const int32_t cc = c0 + lc * CPL + t;
@@ -555,14 +568,19 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
     const auto *uoff = A.lb_uoff.as<int32_t>();
     const auto *ucol = A.lb_ucol.as<int32_t>();
     const auto *lidx = A.lb_lidx.as<uint8_t>();
+    // clustered blocks (nullptr for consecutive rows): row order, block-order
+    // row starts and values
+    const auto *brow = A.lb_brow.as<int32_t>();
+    const auto *prp = A.lb_prp.as<int32_t>();
+    const auto *lval = A.lb_val.as<V>();
     if (beta != 0.0)
       k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
-          row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
-          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
+          row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
+          A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
     else
       k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
-          row0, uoff, ucol, lidx, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(), A.vals.as<V>(),
-          (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
+          row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
+          A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
   };
   // Column chunks: 8 for k <= 8 and for FP64 lists of 256, then 16 (FP64) or
   // 16 / 32 (FP32) columns; at most 16 KB of staged FP64 B, so six workgroups

@@ -520,6 +520,14 @@ __global__ __launch_bounds__(kBlock) void k_patch_vals(int64_t n, const int64_t 
   else vals[slot[i]] = S(v[i]);
 }
 
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_gather_vals(int64_t n, const int32_t *__restrict__ idx,
+                                                        const E *__restrict__ src,
+                                                        E *__restrict__ dst) {
+  const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (q < n) dst[q] = src[idx[q]];
+}
+
 // ===========================================================================
 // Host launchers
 namespace {
@@ -760,6 +768,21 @@ hipError_t axpby_vec(bool dbl, int64_t n, double alpha, const void *x, double be
 hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_f32_to_bf16<<<grid_for(n), kBlock, 0, s>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t gather_vals(VStore vs, void *dst, const void *src, const int32_t *idx, int64_t n,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (vs == VStore::F64)
+    k_gather_vals<uint64_t><<<grid_for(n), kBlock, 0, s>>>(
+        n, idx, static_cast<const uint64_t *>(src), static_cast<uint64_t *>(dst));
+  else if (vs == VStore::F32)
+    k_gather_vals<uint32_t><<<grid_for(n), kBlock, 0, s>>>(
+        n, idx, static_cast<const uint32_t *>(src), static_cast<uint32_t *>(dst));
+  else
+    k_gather_vals<uint16_t><<<grid_for(n), kBlock, 0, s>>>(
+        n, idx, static_cast<const uint16_t *>(src), static_cast<uint16_t *>(dst));
   return hipGetLastError();
 }
 

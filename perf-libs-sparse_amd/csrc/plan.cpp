@@ -192,97 +192,6 @@ static perflibs_status_t build_csr_plan(perflibs_spmat_top_t *A, const Operand &
   return PERFLIBS_STATUS_SUCCESS;
 }
 
-// Greedy cut of consecutive rows into blocks of at most kLbUcols distinct
-// columns, 4 * kLbUcols non-zeros and kLbRows rows; a row that alone exceeds a
-// limit becomes a block without a column list.  Used only when the lists
-// plus the non-zeros of list-less rows are at most 60% of nnz (a B row
-// loaded into LDS then feeds 1.7 non-zeros or more); otherwise the direct
-// kernels, which read B for every non-zero, are used.
-namespace {
-hipError_t cut_spmm_blocks(const Operand &o, DevCsr &d, int32_t kLbUcols) {
-  const int32_t kLbNnz = lb_nnz_per_ucol(kLbUcols) * kLbUcols;
-  const int32_t m = o.m;
-  const int32_t *rp = o.prp, *ci = o.pci;
-  const int32_t base = rp[0];
-  const int64_t nnz = int64_t(rp[m]) - base;
-  const size_t ncols = size_t(std::max(o.n, 1));
-  std::vector<int32_t> row0{0}, uoff{0}, ucol;
-  std::vector<uint8_t> lidx(size_t(std::max<int64_t>(nnz, 1)), 0);
-  // inblk[c] == bid: column c is in the current block's list at pos[c];
-  // seen[c] == pass: column c was counted by the current counting pass
-  std::vector<int32_t> inblk(ncols, -1), pos(ncols, 0), seen(ncols, -1);
-  int32_t bid = 0, pass = 0, u = 0, rows = 0;
-  int64_t bnnz = 0, direct = 0;  // direct: non-zeros of rows without a list
-  auto close = [&](int32_t r) {  // the current block ends before row r
-    row0.push_back(r);
-    uoff.push_back(int32_t(ucol.size()));
-    ++bid;
-    u = rows = 0;
-    bnnz = 0;
-  };
-  auto count_fresh = [&](int64_t j0, int64_t j1) {  // distinct columns not in the list
-    ++pass;
-    int32_t f = 0;
-    for (int64_t j = j0; j < j1; ++j) {
-      const int32_t c = ci[j] - base;
-      if (inblk[size_t(c)] != bid && seen[size_t(c)] != pass) {
-        seen[size_t(c)] = pass;
-        ++f;
-      }
-    }
-    return f;
-  };
-  for (int32_t r = 0; r < m; ++r) {
-    const int64_t j0 = int64_t(rp[r]) - base, j1 = int64_t(rp[r + 1]) - base;
-    int32_t fresh = count_fresh(j0, j1);
-    if (rows > 0 && !(u + fresh <= kLbUcols && bnnz + (j1 - j0) <= kLbNnz && rows < kLbRows)) {
-      close(r);
-      fresh = count_fresh(j0, j1);
-    }
-    if (fresh > kLbUcols || j1 - j0 > kLbNnz) {
-      close(r + 1);  // rows == 0: the row is a block of its own without a list
-      direct += j1 - j0;
-      continue;
-    }
-    for (int64_t j = j0; j < j1; ++j) {
-      const int32_t c = ci[j] - base;
-      if (inblk[size_t(c)] != bid) {
-        inblk[size_t(c)] = bid;
-        pos[size_t(c)] = u++;
-        ucol.push_back(c);
-      }
-      lidx[size_t(j)] = uint8_t(pos[size_t(c)]);
-    }
-    bnnz += j1 - j0;
-    ++rows;
-  }
-  if (rows > 0) close(m);
-  d.lb_nblocks = int32_t(row0.size() - 1);
-  d.lb_ucols = int64_t(ucol.size());
-  d.lb_umax = kLbUcols;
-  if (double(d.lb_ucols + direct) > 0.6 * double(nnz)) {
-    d.lb_state = -1;
-    return hipSuccess;
-  }
-  hipError_t e = d.lb_row0.upload(row0.data(), sizeof(int32_t) * row0.size());
-  if (e == hipSuccess) e = d.lb_uoff.upload(uoff.data(), sizeof(int32_t) * uoff.size());
-  if (e == hipSuccess) e = d.lb_ucol.upload(ucol.data(), sizeof(int32_t) * std::max<size_t>(ucol.size(), 1));
-  if (e == hipSuccess) e = d.lb_lidx.upload(lidx.data(), lidx.size());
-  d.lb_state = e == hipSuccess ? 1 : 0;
-  return e;
-}
-}  // namespace
-
-// Lists of at most 128 columns keep the kernel's LDS small enough for six
-// workgroups per CU (measured 1.3x faster than 256 on the 27-point stencil at
-// k = 32 and 128); lists of 256 serve operands whose rows alone have more
-// distinct columns (e.g. 8 x 8 block rows).
-hipError_t build_spmm_blocks(const Operand &o, DevCsr &d) {
-  hipError_t e = cut_spmm_blocks(o, d, kLbUcolsSmall);
-  if (e == hipSuccess && d.lb_state == -1) e = cut_spmm_blocks(o, d, kLbUcolsLarge);
-  return e;
-}
-
 // The transposed operand's structure of a TRANS / CONJTRANS plan of a CSR or
 // SELL object, kept to locate value updates in place (update.cpp).
 static void keep_transposed_structure(perflibs_spmat_top_t *A, const Operand &o, int op) {

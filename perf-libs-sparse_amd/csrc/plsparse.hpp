@@ -108,19 +108,27 @@ struct DevCsr {
   int32_t nblocks = 0;
   int32_t vec_lanes = 8;  // lanes per row of the CSR-vector kernel
   // Column-compressed row blocks of the wide-B SpMM kernel (built on the first
-  // SpMM with k >= 8): each block of consecutive rows lists its distinct
+  // SpMM with k >= 8, spmm_blocks.cpp): each block of rows lists its distinct
   // columns once (ucol), and each non-zero stores its column's position in
   // that list (lidx, one byte), so a workgroup loads every B row its block
   // needs into LDS once per column chunk.  A block whose row alone exceeds the
-  // limits has no list (ucount 0) and reads B directly.
+  // limits has no list (ucount 0) and reads B directly.  Blocks hold
+  // consecutive rows, or rows clustered by shared columns: then brow lists the
+  // rows block by block, and the blocks' non-zeros are kept in that order too
+  // (prp, lidx, val), so a block reads them as one contiguous range.
   int lb_state = 0;  // 0: not built, 1: built and used, -1: built, too little reuse
   int32_t lb_nblocks = 0;
   int32_t lb_umax = 0;   // list length limit the blocks were cut with
   int64_t lb_ucols = 0;  // total list length (sum of distinct columns per block)
-  DevBuf lb_row0;        // int32 [nb+1] first row of each block
+  bool lb_clustered = false;  // blocks of clustered rows (lb_brow holds them)
+  DevBuf lb_row0;        // int32 [nb+1] block b: entries [row0[b], row0[b+1]) of the row order
   DevBuf lb_uoff;        // int32 [nb+1] offsets into lb_ucol
   DevBuf lb_ucol;        // int32 [lb_ucols]
-  DevBuf lb_lidx;        // uint8 [nnz]
+  DevBuf lb_lidx;        // uint8 [nnz] (block order when clustered)
+  DevBuf lb_brow;        // int32 [m] row order of clustered blocks (empty: row i is entry i)
+  DevBuf lb_prp;         // int32 [m+1] first non-zero of each entry of the row order (clustered)
+  DevBuf lb_epos;        // int32 [nnz] CSR position of each non-zero in block order (clustered)
+  DevBuf lb_val;         // [nnz] vals in block order, storage vs (clustered; refreshed by updates)
 };
 // Limits of a column-compressed SpMM block (must match k_spmm.hip): distinct
 // columns per list (128, or 256 when lists of 128 leave too many rows without
@@ -241,6 +249,7 @@ struct perflibs_spmat_top_t {
   int rebuild_variant = -1;
   int forced_C = -1, forced_sigma = -1;
   int forced_panel_log2 = -1, forced_panel_wpc = -1;  // column-panel shape overrides
+  int spmm_rows = 0;  // SpMM row blocks: 0 the better cut, 1 consecutive rows, 2 clustered rows
   int precision = PERFLIBS_AMD_VALUES_NATIVE;
 
   pls::DevBuf stage_x, stage_y;  // staging of host x / y
@@ -366,8 +375,13 @@ perflibs_status_t build_native_bsr_plan(perflibs_spmat_top_t *A);
 // dense product (out stays empty for any other object).
 perflibs_status_t bsr_blocks_for_spmm(perflibs_spmat_top_t *A, std::unique_ptr<DevBsr> &out);
 // Column-compressed row blocks of the wide-B SpMM kernel for the CSR plan `d`
-// of operand `o` (sets d.lb_state; a HIP error is returned as is).
-hipError_t build_spmm_blocks(const Operand &o, DevCsr &d);
+// of operand `o` (sets d.lb_state; a HIP error is returned as is).  rows: 0
+// keeps the cut with the shorter lists, 1 cuts consecutive rows only, 2 takes
+// the clustered cut whenever it completes.
+hipError_t build_spmm_blocks(const Operand &o, DevCsr &d, int rows = 0);
+// Clustered blocks: lb_val[q] = vals[lb_epos[q]] (after the build and after
+// value updates), enqueued on the library stream.
+hipError_t refresh_block_values(DevCsr &d);
 // The device plan an SpMV with `op` runs (built on demand, src/matvec.cpp:122-129).
 perflibs_status_t ensure_plan(perflibs_spmat_top_t *A, int op);
 // Launch the current plan: y = alpha*A*x + beta*y on device vectors.
@@ -460,6 +474,9 @@ hipError_t f32_to_bf16(const float *src, uint16_t *dst, int64_t n, hipStream_t s
 // or carry equal values.
 hipError_t patch_vals(VStore vs, void *vals, const int64_t *slot, const void *v, int64_t n,
                       hipStream_t s);
+// dst[q] = src[idx[q]] for q < n, elements of the storage type vs.
+hipError_t gather_vals(VStore vs, void *dst, const void *src, const int32_t *idx, int64_t n,
+                       hipStream_t s);
 // SELL-C-sigma chunk widths cl[nC] and chunk_min_nnz[nC] of an m-row 0-based
 // device CSR (C and sigma powers of two, sigma 0 or in [C, 16384]; else
 // hipErrorInvalidValue).

@@ -192,6 +192,8 @@ bool patch_plan(perflibs_spmat_top_t *A, const std::vector<ValueHit> &hits) {
   hipError_t e = dslot.upload(slot.data(), sizeof(int64_t) * nh);
   if (e == hipSuccess) e = dval.upload(src, (dbl ? 8 : 4) * nh);
   if (e == hipSuccess) e = plk::patch_vals(vs, dvals, dslot.as<int64_t>(), dval.get(), int64_t(nh), stream());
+  // clustered SpMM blocks keep a block-order copy of the CSR plan's values
+  if (e == hipSuccess && pl.csr) e = refresh_block_values(*pl.csr);
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
   return e == hipSuccess;
 }

@@ -116,6 +116,8 @@ _SIGS = {
     "perflibs_amd_force_panel": (_I, [_P, _I, _I]),
     "perflibs_amd_panel_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "perflibs_amd_spmm_info": (_I, [_P, _P, _P, _P, _P]),
+    "perflibs_amd_set_spmm_rows": (_I, [_P, _I]),
+    "perflibs_amd_spmm_rows_clustered": (_I, [_P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
     "perflibs_amd_sell_beta": (_I, [_P, _I, _I, _I, _P, _P, _P]),
@@ -338,6 +340,16 @@ class SpMat:
         check("spmm_info", lib().perflibs_amd_spmm_info(self.h, *[C.byref(a) for a in v], C.byref(tot)))
         return {"blocks_state": v[0].value, "nblocks": v[1].value, "list_max": v[2].value,
                 "list_total": tot.value}
+
+    def set_spmm_rows(self, rows: int):
+        """0: the better of consecutive and clustered row blocks; 1: consecutive
+        only; 2: clustered whenever the greedy completes (next SpMM, k >= 8)."""
+        check("set_spmm_rows", lib().perflibs_amd_set_spmm_rows(self.h, rows))
+
+    def spmm_rows_clustered(self) -> bool:
+        v = C.c_int()
+        check("spmm_rows_clustered", lib().perflibs_amd_spmm_rows_clustered(self.h, C.byref(v)))
+        return bool(v.value)
 
     def panel_info(self) -> dict:
         v = [C.c_int32() for _ in range(5)]
