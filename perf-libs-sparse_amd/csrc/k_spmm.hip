@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 // a B row feeds about 2.7 non-zeros of the block, so L2 -> CU traffic for B
 // falls by that factor against the direct kernels, which fetch a B row per
 // non-zero.  A block without a list (one row with more than UMAX distinct
-// columns or 4 * UMAX non-zeros) spreads its non-zeros over the workgroup's
+// columns or lb_nnz_per_ucol(UMAX) * UMAX non-zeros) spreads its non-zeros over the workgroup's
 // lane groups, reads B directly and sums the groups through LDS.
 template <int KC, int UMAX, int CPL, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
