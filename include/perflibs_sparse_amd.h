@@ -131,6 +131,19 @@ perflibs_status_t perflibs_amd_set_spmm_rows(perflibs_spmat_t A, int rows);
 /* *clustered = 1 when the built blocks hold clustered rows, 0 for consecutive
  * rows; status 1 before the blocks are built. */
 perflibs_status_t perflibs_amd_spmm_rows_clustered(perflibs_spmat_t A, int *clustered);
+/* The row blocks an SpMM plan would cut for an m x n CSR pattern (row_ptr[0]
+ * = index base 0 or 1), computed on the host with no device or handle: *used
+ * = 1 when the blocks give enough B-row reuse to be used; nblocks, list_max,
+ * list_total and clustered as above; row_order[m] (0-based rows, block by
+ * block) and block_starts[nblocks + 1] (offsets into row_order; give room for
+ * m + 1) are filled when not null.  rows as for perflibs_amd_set_spmm_rows. */
+perflibs_status_t perflibs_amd_spmm_blocks_inspect(perflibs_int_t m, perflibs_int_t n,
+                                                  const perflibs_int_t *row_ptr,
+                                                  const perflibs_int_t *col_indx, int rows,
+                                                  int *used, perflibs_int_t *nblocks,
+                                                  perflibs_int_t *list_max, int64_t *list_total,
+                                                  int *clustered, perflibs_int_t *row_order,
+                                                  perflibs_int_t *block_starts);
 /* Shape of a column-panel plan (for tests and tuning). */
 perflibs_status_t perflibs_amd_panel_info(perflibs_spmat_t A, perflibs_int_t *panel_log2,
                                           perflibs_int_t *npanels, perflibs_int_t *ntiles,

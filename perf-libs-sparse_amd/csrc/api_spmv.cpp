@@ -318,6 +318,34 @@ perflibs_status_t perflibs_amd_set_spmm_rows(perflibs_spmat_t A, int rows) {
   return PERFLIBS_STATUS_SUCCESS;
 }
 
+perflibs_status_t perflibs_amd_spmm_blocks_inspect(perflibs_int_t m, perflibs_int_t n,
+                                                  const perflibs_int_t *row_ptr,
+                                                  const perflibs_int_t *col_indx, int rows,
+                                                  int *used, perflibs_int_t *nblocks,
+                                                  perflibs_int_t *list_max, int64_t *list_total,
+                                                  int *clustered, perflibs_int_t *row_order,
+                                                  perflibs_int_t *block_starts) {
+  if (m <= 0 || n <= 0 || !row_ptr || !col_indx || rows < 0 || rows > 2)
+    return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  const perflibs_int_t base = row_ptr[0];
+  if (base != 0 && base != 1) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  for (perflibs_int_t i = 0; i < m; ++i)
+    if (row_ptr[i + 1] < row_ptr[i]) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  for (int64_t j = 0; j < int64_t(row_ptr[m]) - base; ++j)
+    if (col_indx[j] < base || col_indx[j] - base >= n) return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;
+  pls::Operand o;
+  o.m = m;
+  o.n = n;
+  o.prp = row_ptr;
+  o.pci = col_indx;
+  int32_t c = 0;
+  const int r = pls::spmm_blocks_host(o, rows, nblocks, list_max, list_total, &c, row_order,
+                                      block_starts);
+  if (used) *used = r == 1 ? 1 : 0;
+  if (clustered) *clustered = c;
+  return PERFLIBS_STATUS_SUCCESS;
+}
+
 perflibs_status_t perflibs_amd_spmm_rows_clustered(perflibs_spmat_t A, int *clustered) {
   if (!A || !clustered || !A->plan.csr || A->plan.csr->lb_state == 0)
     return PERFLIBS_STATUS_INPUT_PARAMETER_ERROR;

@@ -381,6 +381,12 @@ perflibs_status_t bsr_blocks_for_spmm(perflibs_spmat_top_t *A, std::unique_ptr<D
 // keeps the cut with the shorter lists, 1 cuts consecutive rows only, 2 takes
 // the clustered cut whenever it completes.
 hipError_t build_spmm_blocks(const Operand &o, DevCsr &d, int rows = 0);
+// The same choice on the host only, for inspection: returns 1 when the blocks
+// would be used, -1 when none gives enough reuse (the last cut tried is
+// described then); row_order[m] and block_starts[nblocks + 1] may be null.
+int spmm_blocks_host(const Operand &o, int rows, int32_t *nblocks, int32_t *list_max,
+                     int64_t *list_total, int32_t *clustered, int32_t *row_order,
+                     int32_t *block_starts);
 // Clustered blocks: lb_val[q] = vals[lb_epos[q]] (after the build and after
 // value updates), enqueued on the library stream.
 hipError_t refresh_block_values(DevCsr &d);

@@ -401,7 +401,10 @@ hipError_t upload_blocks(const Operand &o, const LbHost &h, int32_t umax, DevCsr
 // lists plus the non-zeros of list-less rows are at most 60% of nnz (a B row
 // loaded into LDS then feeds 1.7 non-zeros or more); otherwise the direct
 // kernels, which read B for every non-zero, run.
-hipError_t build_spmm_blocks(const Operand &o, DevCsr &d, int rows) {
+namespace {
+// The blocks the plan uses (true), or the last cut tried when no list length
+// limit gives enough reuse (false); umax = the limit of the returned cut.
+bool choose_blocks(const Operand &o, int rows, LbHost &out, int32_t &umax_out) {
   const int64_t nnz = int64_t(o.prp[o.m]) - o.prp[0];
   std::vector<int32_t> cp, cr;
   int overlap = -1;     // 1: rows share enough columns for clustering to pay
@@ -423,14 +426,43 @@ hipError_t build_spmm_blocks(const Operand &o, DevCsr &d, int rows) {
           (rows == 2 || double(c.cost()) < 0.9 * double(h.cost())))
         h = std::move(c);
     }
-    d.lb_nblocks = int32_t(h.row0.size() - 1);
-    d.lb_ucols = int64_t(h.ucol.size());
-    d.lb_umax = umax;
-    d.lb_clustered = !h.brow.empty();
-    if (double(h.cost()) <= 0.6 * double(nnz)) return upload_blocks(o, h, umax, d);
+    const bool used = double(h.cost()) <= 0.6 * double(nnz);
+    out = std::move(h);
+    umax_out = umax;
+    if (used) return true;
   }
+  return false;
+}
+}  // namespace
+
+hipError_t build_spmm_blocks(const Operand &o, DevCsr &d, int rows) {
+  LbHost h;
+  int32_t umax = 0;
+  const bool used = choose_blocks(o, rows, h, umax);
+  d.lb_nblocks = int32_t(h.row0.size() - 1);
+  d.lb_ucols = int64_t(h.ucol.size());
+  d.lb_umax = umax;
+  d.lb_clustered = !h.brow.empty();
+  if (used) return upload_blocks(o, h, umax, d);
   d.lb_state = -1;
   return hipSuccess;
+}
+
+int spmm_blocks_host(const Operand &o, int rows, int32_t *nblocks, int32_t *list_max,
+                     int64_t *list_total, int32_t *clustered, int32_t *row_order,
+                     int32_t *block_starts) {
+  LbHost h;
+  int32_t umax = 0;
+  const bool used = choose_blocks(o, rows, h, umax);
+  const int32_t nb = int32_t(h.row0.size() - 1);
+  if (nblocks) *nblocks = nb;
+  if (list_max) *list_max = umax;
+  if (list_total) *list_total = int64_t(h.ucol.size());
+  if (clustered) *clustered = h.brow.empty() ? 0 : 1;
+  if (row_order)
+    for (int32_t i = 0; i < o.m; ++i) row_order[i] = h.brow.empty() ? i : h.brow[size_t(i)];
+  if (block_starts) std::copy(h.row0.begin(), h.row0.end(), block_starts);
+  return used ? 1 : -1;
 }
 
 hipError_t refresh_block_values(DevCsr &d) {

@@ -118,6 +118,7 @@ _SIGS = {
     "perflibs_amd_spmm_info": (_I, [_P, _P, _P, _P, _P]),
     "perflibs_amd_set_spmm_rows": (_I, [_P, _I]),
     "perflibs_amd_spmm_rows_clustered": (_I, [_P, _P]),
+    "perflibs_amd_spmm_blocks_inspect": (_I, [_I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "perflibs_amd_features": (_I, [_P, _P]),
     "perflibs_amd_time_variants": (_I, [_P, C.c_int, C.c_int, _P]),
     "perflibs_amd_sell_beta": (_I, [_P, _I, _I, _I, _P, _P, _P]),
@@ -556,6 +557,24 @@ def spmm_exec(transA, transB, alpha, A: SpMat, B: SpMat, beta, Cm: SpMat, raise_
     if raise_on_error:
         check(fn, st)
     return st
+
+
+def spmm_blocks_inspect(m: int, n: int, row_ptr, col_indx, rows: int = 0) -> dict:
+    """The row blocks an SpMM plan would cut for this CSR pattern (host only, no
+    device): used, nblocks, list_max, list_total, clustered, row_order (rows
+    block by block) and block_starts (offsets into row_order)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_indx, dtype=np.int32)
+    order = np.empty(max(m, 1), np.int32)
+    starts = np.empty(max(m, 1) + 1, np.int32)
+    used, nb, lm, cl = C.c_int(), C.c_int32(), C.c_int32(), C.c_int()
+    tot = C.c_int64()
+    check("spmm_blocks_inspect", lib().perflibs_amd_spmm_blocks_inspect(
+        m, n, _ptr(rp), _ptr(ci), rows, C.byref(used), C.byref(nb), C.byref(lm), C.byref(tot),
+        C.byref(cl), _ptr(order), _ptr(starts)))
+    return {"used": bool(used.value), "nblocks": nb.value, "list_max": lm.value,
+            "list_total": tot.value, "clustered": bool(cl.value), "row_order": order[:m],
+            "block_starts": starts[: nb.value + 1]}
 
 
 def spmm_optimize(transA, transB, alpha_hint, A: SpMat, B: SpMat, beta_hint, Cm: SpMat,
