@@ -41,13 +41,32 @@ SellParams sell_params(int variant) {
 bool is_sell(int v) { return v >= PERFLIBS_AMD_SELL_C32_S0 && v <= PERFLIBS_AMD_SELL_C64_S16384; }
 
 // Upload `n` values of the handle dtype, converting to the storage type.
-hipError_t upload_vals(DevBuf &dst, const void *host, int64_t n, DType dt, VStore vs) {
-  if (vs == VStore::F64) return dst.upload(host, size_t(n) * 8);
-  if (vs == VStore::F32) return dst.upload(host, size_t(n) * 4);
+// Zero bytes copied in after an array (synchronous copies, ordered before any
+// later launch on any stream).
+hipError_t zero_tail(void *dst, size_t bytes) {
+  const std::vector<unsigned char> z(bytes, 0);
+  return bytes ? hipMemcpy(dst, z.data(), bytes, hipMemcpyHostToDevice) : hipSuccess;
+}
+
+// n elements of es bytes from the host followed by `pad` zero elements.
+hipError_t upload_padded(DevBuf &dst, const void *host, size_t n, size_t es, size_t pad) {
+  hipError_t e = dst.alloc((n + pad) * es);
+  if (e == hipSuccess && n > 0) e = hipMemcpy(dst.get(), host, n * es, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = zero_tail(static_cast<char *>(dst.get()) + n * es, pad * es);
+  return e;
+}
+
+// Values in the storage type vs, followed by `pad` zero values.
+hipError_t upload_vals(DevBuf &dst, const void *host, int64_t n, DType dt, VStore vs,
+                       int64_t pad = 0) {
+  if (vs == VStore::F64) return upload_padded(dst, host, size_t(n), 8, size_t(pad));
+  if (vs == VStore::F32) return upload_padded(dst, host, size_t(n), 4, size_t(pad));
   DevBuf tmp;
   hipError_t e = tmp.upload(host, size_t(n) * 4);
   if (e != hipSuccess) return e;
-  e = dst.alloc(size_t(n) * 2);
+  e = dst.alloc(size_t(n + pad) * 2);
+  if (e != hipSuccess) return e;
+  e = zero_tail(dst.as<uint16_t>() + n, size_t(pad) * 2);
   if (e != hipSuccess) return e;
   e = plk::f32_to_bf16(tmp.as<float>(), dst.as<uint16_t>(), n, stream());
   if (e != hipSuccess) return e;
@@ -160,8 +179,10 @@ static perflibs_status_t build_csr_plan(perflibs_spmat_top_t *A, const Operand &
   d->vs = value_store(A);
   d->vec_lanes = vector_lanes(o.m, d->nnz);
   hipError_t e = d->rp.upload(o.prp, sizeof(int32_t) * (size_t(o.m) + 1));
-  if (e == hipSuccess) e = d->ci.upload(o.pci, sizeof(int32_t) * size_t(d->nnz));
-  if (e == hipSuccess) e = upload_vals(d->vals, o.pvals, d->nnz, A->dt, d->vs);
+  // one padding entry (column 0, value 0) after the last: the adaptive kernel
+  // reads entries in aligned pairs
+  if (e == hipSuccess) e = upload_padded(d->ci, o.pci, size_t(d->nnz), sizeof(int32_t), 1);
+  if (e == hipSuccess) e = upload_vals(d->vals, o.pvals, d->nnz, A->dt, d->vs, 1);
   if (e == hipSuccess) {
     // nnz-balanced row blocks for the adaptive kernel
     std::vector<int32_t> blocks{0};
