@@ -190,38 +190,70 @@ __global__ __launch_bounds__(kBlock) void k_sell(
     X *__restrict__ y, acc_t<X> alpha, acc_t<X> beta) {
   using A = acc_t<X>;
   const int C = CT ? CT : Crt;
+  // rows per lane: two adjacent rows of a chunk when C is 32 or 64 and the
+  // rows are not sorted (sigma = 0), so a lane's slot pair is one load of 2
+  // offsets and one of 2 values (chunk offsets are multiples of C, hence
+  // even): 7-16% faster on the 27-point stencil and on banded rows; with
+  // sorted rows the same change was 2-39% slower, so those keep one row per
+  // lane, as do other C
+  constexpr int RPL = CT && !PERM ? 2 : 1;
   const int64_t t = int64_t(xcd_swizzle(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
-  const int64_t c = t / C;
-  const int r = int(t - c * C);
+  const int64_t c = t / (C / RPL);
+  const int r = int(t - c * (C / RPL)) * RPL;
   if (c >= nC) return;
   const int64_t base = cs[c] + r;
   const int32_t w = cl[c];
   const X *xm = x + cmin[c];
-  // U slots per step: U offset and U value loads issue back to back, then
-  // the U gathers; the tail step clamps to the last slot and masks it.
-  constexpr int U = 8;
-  A s = 0;
+  // U slots per step: the offset and value loads issue back to back, then
+  // the gathers; the tail step clamps to the last slot and masks it.
+  constexpr int U = 8 / RPL;
+  A s[RPL];
+#pragma unroll
+  for (int h = 0; h < RPL; ++h) s[h] = 0;
   for (int32_t j = 0; j < w; j += U) {
-    uint32_t o[U];
-    A a[U];
+    uint32_t o[U][RPL];
+    A a[U][RPL];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t at = base + int64_t(min(j + k, w - 1)) * C;
-      o[k] = ldo(offs, at);
-      a[k] = A(ldv(val, at));
+      if constexpr (RPL == 2) {
+        typedef OFF O2 __attribute__((ext_vector_type(2)));
+        const O2 oo = __builtin_nontemporal_load(reinterpret_cast<const O2 *>(offs + at));
+        o[k][0] = oo.x;
+        o[k][1] = oo.y;
+        if constexpr (sizeof(V) == 2) {
+          typedef uint16_t V2 __attribute__((ext_vector_type(2)));
+          const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + at));
+          a[k][0] = A(__uint_as_float(uint32_t(vv.x) << 16));
+          a[k][1] = A(__uint_as_float(uint32_t(vv.y) << 16));
+        } else {
+          typedef V V2 __attribute__((ext_vector_type(2)));
+          const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + at));
+          a[k][0] = A(vv.x);
+          a[k][1] = A(vv.y);
+        }
+      } else {
+        o[k][0] = ldo(offs, at);
+        a[k][0] = A(ldv(val, at));
+      }
     }
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const A p = a[k] * A(xm[o[k]]);
-      s += j + k < w ? p : A(0);
-    }
+    for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int h = 0; h < RPL; ++h) {
+        const A p = a[k][h] * A(xm[o[k][h]]);
+        s[h] += j + k < w ? p : A(0);
+      }
   }
-  const int64_t pr = c * C + r;
-  const int32_t row = PERM ? perm[pr] : int32_t(pr);
-  if (PERM ? row >= 0 : pr < m) {
-    A res = alpha * s;
-    if (BETA) res += beta * A(y[row]);
-    y[row] = X(res);
+#pragma unroll
+  for (int h = 0; h < RPL; ++h) {
+    const int64_t pr = c * C + r + h;
+    const int32_t row = PERM ? perm[pr] : int32_t(pr);
+    if (PERM ? row >= 0 : pr < m) {
+      A res = alpha * s[h];
+      if (BETA) res += beta * A(y[row]);
+      y[row] = X(res);
+    }
   }
 }
 
@@ -711,9 +743,9 @@ namespace {
 template <int CT, class OFF, class V, class X>
 void launch_sell(const pls::DevSell &A, double alpha, const void *x, double beta, void *y,
                  hipStream_t s) {
-  const unsigned grid = grid_for(A.nC * A.C);
-  if (grid == 0) return;
   const bool perm = A.sigma != 0;
+  const unsigned grid = grid_for(A.nC * A.C / (CT && !perm ? 2 : 1));  // k_sell's rows per lane
+  if (grid == 0) return;
   const auto *cs = A.cs.as<int64_t>();
   const auto *cl = A.cl.as<int32_t>();
   const auto *cm = A.cmin.as<int32_t>();
