@@ -16,6 +16,26 @@ namespace plk {
 // reference's row-parallel kernels K1/K2 (src/compressed_sparse_rows.cpp:
 // 1114-1138, 1215-1321): same per-row result, the row's products split over L
 // lanes and combined by a shuffle tree instead of two interleaved partials.
+// Entries j and j + 1 (j even): column indices as one 8-byte load, values as
+// one 2 * sizeof(V)-byte load, both non-temporal.
+template <class V, class A>
+__device__ __forceinline__ void ld_pair(const int32_t *__restrict__ ci, const V *__restrict__ val,
+                                        int32_t j, int32_t (&c)[2], A (&a)[2]) {
+  typedef int32_t I2 __attribute__((ext_vector_type(2)));
+  typedef V V2 __attribute__((ext_vector_type(2)));
+  const I2 ic = __builtin_nontemporal_load(reinterpret_cast<const I2 *>(ci + j));
+  const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + j));
+  c[0] = ic.x;
+  c[1] = ic.y;
+  if constexpr (sizeof(V) == 2) {  // BF16: the upper half of an FP32 word
+    a[0] = A(__uint_as_float(uint32_t(vv.x) << 16));
+    a[1] = A(__uint_as_float(uint32_t(vv.y) << 16));
+  } else {
+    a[0] = A(vv.x);
+    a[1] = A(vv.y);
+  }
+}
+
 template <int L, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_csr_vector(
     int32_t m, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -27,13 +47,24 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(
   const int lane = threadIdx.x & (L - 1);
   A s = 0;
   if (row < m) {
-    const int32_t e = rp[row + 1];
-    constexpr int U = 4;
-    for (int32_t j = rp[row] + lane; j < e; j += U * L) {
-      A p[U];
-      gather_products<U>(ci, val, x, j, L, e - 1, p);
+    // the row's entries in aligned pairs from the even index at or below its
+    // first (the plan's arrays end with one padding entry); pairs outside
+    // [rs, e) are masked
+    const int32_t rs = rp[row], e = rp[row + 1];
+    const int32_t last = (e - 1) & ~1;
+    constexpr int U = 2;  // pairs in flight per lane
+    for (int32_t j = (rs & ~1) + 2 * lane; j <= last && rs < e; j += 2 * U * L) {
+      int32_t c[U][2];
+      A a[U][2];
 #pragma unroll
-      for (int k = 0; k < U; ++k) s += j + k * L < e ? p[k] : A(0);
+      for (int u = 0; u < U; ++u) ld_pair(ci, val, min(j + 2 * u * L, last), c[u], a[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t q = j + 2 * u * L;
+        const A p0 = a[u][0] * A(x[c[u][0]]), p1 = a[u][1] * A(x[c[u][1]]);
+        s += q >= rs && q < e ? p0 : A(0);
+        s += q + 1 >= rs && q + 1 < e ? p1 : A(0);
+      }
     }
   }
   s = group_sum_c<L>(s);
@@ -57,26 +88,6 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(
 // row is finished by exactly one workgroup.
 constexpr int kAdaVpt = 8;                  // non-zeros staged per thread
 constexpr int kAdaNnz = kBlock * kAdaVpt;   // non-zeros per row block
-
-// Entries j and j + 1 (j even): column indices as one 8-byte load, values as
-// one 2 * sizeof(V)-byte load, both non-temporal.
-template <class V, class A>
-__device__ __forceinline__ void ld_pair(const int32_t *__restrict__ ci, const V *__restrict__ val,
-                                        int32_t j, int32_t (&c)[2], A (&a)[2]) {
-  typedef int32_t I2 __attribute__((ext_vector_type(2)));
-  typedef V V2 __attribute__((ext_vector_type(2)));
-  const I2 ic = __builtin_nontemporal_load(reinterpret_cast<const I2 *>(ci + j));
-  const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + j));
-  c[0] = ic.x;
-  c[1] = ic.y;
-  if constexpr (sizeof(V) == 2) {  // BF16: the upper half of an FP32 word
-    a[0] = A(__uint_as_float(uint32_t(vv.x) << 16));
-    a[1] = A(__uint_as_float(uint32_t(vv.y) << 16));
-  } else {
-    a[0] = A(vv.x);
-    a[1] = A(vv.y);
-  }
-}
 
 template <class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_csr_adaptive(
@@ -708,7 +719,8 @@ hipError_t csr_vector(const pls::DevCsr &A, bool dbl, double alpha, const void *
   return with_types(A.vs, dbl, [&](auto vtag, auto xtag) -> hipError_t {
     using V = std::remove_const_t<std::remove_pointer_t<decltype(vtag)>>;
     using X = std::remove_pointer_t<decltype(xtag)>;
-    switch (A.vec_lanes) {
+    // a lane reads entry pairs: half the lanes per row of one-entry lanes
+    switch (std::max(A.vec_lanes / 2, 1)) {
       case 1: launch_vec<1, V, X>(A, alpha, x, beta, y, s); break;
       case 2: launch_vec<2, V, X>(A, alpha, x, beta, y, s); break;
       case 4: launch_vec<4, V, X>(A, alpha, x, beta, y, s); break;
