@@ -384,13 +384,28 @@ __global__ __launch_bounds__(kBlock) void k_csr_few_rhs(
 #pragma unroll
   for (int r = 0; r < NR; ++r) s[r] = A(0);
   if (row < m) {
-    const int32_t e = rp[row + 1];
-    for (int32_t j = rp[row] + lane; j < e; j += L) {
-      const A a = A(ldv(val, j));
-      A b[P];
-      Piece<P, X>::load(B + int64_t(ldi(ci, j)) * P, b);
+    // the row's entries in aligned pairs (the CSR plan's arrays end with one
+    // padding entry), two pairs in flight per lane; entries outside the row
+    // multiply by zero
+    const int32_t rs = rp[row], e = rp[row + 1];
+    const int32_t last = (e - 1) & ~1;
+    constexpr int U = 2;
+    for (int32_t j = (rs & ~1) + 2 * lane; j <= last && rs < e; j += 2 * U * L) {
+      int32_t c[U][2];
+      A a[U][2];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) s[r] += a * b[r];
+      for (int u = 0; u < U; ++u) ld_pair(ci, val, min(j + 2 * u * L, last), c[u], a[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int32_t q = j + 2 * u * L + h;
+          const A w = q >= rs && q < e ? a[u][h] : A(0);
+          A b[P];
+          Piece<P, X>::load(B + int64_t(c[u][h]) * P, b);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) s[r] += w * b[r];
+        }
     }
   }
 #pragma unroll
@@ -674,7 +689,8 @@ void launch_few(const pls::DevCsr &A, double alpha, const void *B, double beta, 
 template <int NR, int P, class V, class X>
 void few_lanes(const pls::DevCsr &A, double alpha, const void *B, double beta, void *C,
                int64_t c_sr, int64_t c_sc, hipStream_t s) {
-  switch (A.vec_lanes) {
+  // a lane reads entry pairs: half the lanes per row of one-entry lanes
+  switch (std::max(A.vec_lanes / 2, 1)) {
     case 1: launch_few<1, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
     case 2: launch_few<2, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;
     case 4: launch_few<4, NR, P, V, X>(A, alpha, B, beta, C, c_sr, c_sc, s); break;

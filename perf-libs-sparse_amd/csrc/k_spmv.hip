@@ -16,26 +16,6 @@ namespace plk {
 // reference's row-parallel kernels K1/K2 (src/compressed_sparse_rows.cpp:
 // 1114-1138, 1215-1321): same per-row result, the row's products split over L
 // lanes and combined by a shuffle tree instead of two interleaved partials.
-// Entries j and j + 1 (j even): column indices as one 8-byte load, values as
-// one 2 * sizeof(V)-byte load, both non-temporal.
-template <class V, class A>
-__device__ __forceinline__ void ld_pair(const int32_t *__restrict__ ci, const V *__restrict__ val,
-                                        int32_t j, int32_t (&c)[2], A (&a)[2]) {
-  typedef int32_t I2 __attribute__((ext_vector_type(2)));
-  typedef V V2 __attribute__((ext_vector_type(2)));
-  const I2 ic = __builtin_nontemporal_load(reinterpret_cast<const I2 *>(ci + j));
-  const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + j));
-  c[0] = ic.x;
-  c[1] = ic.y;
-  if constexpr (sizeof(V) == 2) {  // BF16: the upper half of an FP32 word
-    a[0] = A(__uint_as_float(uint32_t(vv.x) << 16));
-    a[1] = A(__uint_as_float(uint32_t(vv.y) << 16));
-  } else {
-    a[0] = A(vv.x);
-    a[1] = A(vv.y);
-  }
-}
-
 template <int L, class V, class X, bool BETA>
 __global__ __launch_bounds__(kBlock) void k_csr_vector(
     int32_t m, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
