@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 // non-zero.  A block without a list (one row with more than UMAX distinct
 // columns or lb_nnz_per_ucol(UMAX) * UMAX non-zeros) spreads its non-zeros over the workgroup's
 // lane groups, reads B directly and sums the groups through LDS.
-template <int KC, int UMAX, int CPL, class V, class X, bool BETA>
+template <int KC, int UMAX, int CPL, class V, class X, bool BETA, bool VB>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
     const int32_t *__restrict__ ucol, const uint8_t *__restrict__ lidx,
@@ -258,19 +258,47 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
   while (S * 2 * G <= 64 && nrows * S * 2 <= NGC) S *= 2;
   const int lc = tid % G, sub = (tid % (G * S)) / G;
   for (int32_t c0 = 0; c0 < k; c0 += KC) {
-    const int32_t col = c0 + lane;
-    // B rows of the list, columns [c0, c0 + KC): NG rows per pass, NIF passes
-    // of loads in flight before their LDS stores
-    for (int32_t i0 = grp; i0 < nu; i0 += NIF * NG) {
-      A t[NIF];
+    if constexpr (VB) {
+      // B rows of the list, columns [c0, c0 + KC), 16 bytes per lane (B and
+      // its rows 16-byte aligned): half (FP64) or a quarter (FP32) of the
+      // load instructions of one element per lane
+      constexpr int VW = 16 / sizeof(X), LR = KC / VW, NGV = kBlock / LR;
+      constexpr int NIFV = UMAX / NGV < 1 ? 1 : UMAX / NGV < 8 ? UMAX / NGV : 8;
+      typedef X XV __attribute__((ext_vector_type(VW)));
+      const int lv = tid % LR;
+      const int32_t cv = c0 + lv * VW;
+      const bool full = cv + VW <= k;
+      for (int32_t i0 = tid / LR; i0 < nu; i0 += NIFV * NGV) {
+        XV t[NIFV];
 #pragma unroll
-      for (int s = 0; s < NIF; ++s) {
-        const int32_t i = min(i0 + s * NG, nu - 1);
-        t[s] = col < k ? A(B[int64_t(s_uc[i]) * ldb + col]) : A(0);
+        for (int s = 0; s < NIFV; ++s) {
+          const X *src = B + int64_t(s_uc[min(i0 + s * NGV, nu - 1)]) * ldb + cv;
+          if (full) {
+            t[s] = *reinterpret_cast<const XV *>(src);
+          } else {
+#pragma unroll
+            for (int e = 0; e < VW; ++e) t[s][e] = cv + e < k ? src[e] : X(0);
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < NIFV; ++s)
+          if (i0 + s * NGV < nu) *reinterpret_cast<XV *>(&sB[(i0 + s * NGV) * KC + lv * VW]) = t[s];
       }
+    } else {
+      const int32_t col = c0 + lane;
+      // B rows of the list, columns [c0, c0 + KC): NG rows per pass, NIF passes
+      // of loads in flight before their LDS stores
+      for (int32_t i0 = grp; i0 < nu; i0 += NIF * NG) {
+        A t[NIF];
 #pragma unroll
-      for (int s = 0; s < NIF; ++s)
-        if (i0 + s * NG < nu) sB[(i0 + s * NG) * KC + lane] = t[s];
+        for (int s = 0; s < NIF; ++s) {
+          const int32_t i = min(i0 + s * NG, nu - 1);
+          t[s] = col < k ? A(B[int64_t(s_uc[i]) * ldb + col]) : A(0);
+        }
+#pragma unroll
+        for (int s = 0; s < NIF; ++s)
+          if (i0 + s * NG < nu) sB[(i0 + s * NG) * KC + lane] = t[s];
+      }
     }
     __syncthreads();
     for (int32_t rr = tid / (G * S); rr < nrows; rr += kBlock / (G * S)) {
@@ -588,14 +616,21 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
     const auto *brow = A.lb_brow.as<int32_t>();
     const auto *prp = A.lb_prp.as<int32_t>();
     const auto *lval = A.lb_val.as<V>();
-    if (beta != 0.0)
-      k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, true><<<A.lb_nblocks, kBlock, 0, s>>>(
-          row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
-          A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
-    else
-      k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, false><<<A.lb_nblocks, kBlock, 0, s>>>(
-          row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
-          A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
+    // 16-byte B staging loads when B and its rows are 16-byte aligned
+    const bool vb = (reinterpret_cast<uintptr_t>(B) & 15) == 0 && (ldb * int64_t(sizeof(X))) % 16 == 0;
+    auto launch = [&](auto vtag) {
+      constexpr bool VB = decltype(vtag)::value;
+      if (beta != 0.0)
+        k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, true, VB><<<A.lb_nblocks, kBlock, 0, s>>>(
+            row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
+            A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(beta));
+      else
+        k_spmm_lds<KC, UMAX, (KC >= 16 ? 4 : 2), V, X, false, VB><<<A.lb_nblocks, kBlock, 0, s>>>(
+            row0, uoff, ucol, lidx, brow, prp, lval, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
+            A.vals.as<V>(), (const X *)B, ldb, (X *)C, ldc, acc_t<X>(alpha), acc_t<X>(0));
+    };
+    if (vb) launch(std::true_type{});
+    else launch(std::false_type{});
   };
   // Column chunks: 8 for k <= 8 and for FP64 lists of 256, then 16 (FP64) or
   // 16 / 32 (FP32) columns; at most 16 KB of staged FP64 B, so six workgroups
