@@ -336,13 +336,27 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
         for (int o = G; o < G * S; o <<= 1) acc[t] += __shfl_xor(acc[t], o);
       if (sub == 0) {
         X *c = Cm + int64_t(brow ? s_row[rr] : r0 + rr) * ldc;
+        const int32_t cb = c0 + lc * CPL;
+        if (VB && cb + CPL <= k) {  // the lane's CPL columns as one vector (C aligned like B)
+          typedef X XC __attribute__((ext_vector_type(CPL)));
+          XC o;
+          if (BETA) o = *reinterpret_cast<const XC *>(c + cb);
 #pragma unroll
-        for (int t = 0; t < CPL; ++t) {
-          const int32_t cc = c0 + lc * CPL + t;
-          if (cc < k) {
+          for (int t = 0; t < CPL; ++t) {
             A r = acc[t];
-            if (BETA) r += beta * A(c[cc]);
-            c[cc] = X(r);
+            if (BETA) r += beta * A(o[t]);
+            o[t] = X(r);
+          }
+          *reinterpret_cast<XC *>(c + cb) = o;
+        } else {
+#pragma unroll
+          for (int t = 0; t < CPL; ++t) {
+            const int32_t cc = cb + t;
+            if (cc < k) {
+              A r = acc[t];
+              if (BETA) r += beta * A(c[cc]);
+              c[cc] = X(r);
+            }
           }
         }
       }
@@ -616,8 +630,10 @@ void launch_lds(const pls::DevCsr &A, int32_t k, double alpha, const void *B, in
     const auto *brow = A.lb_brow.as<int32_t>();
     const auto *prp = A.lb_prp.as<int32_t>();
     const auto *lval = A.lb_val.as<V>();
-    // 16-byte B staging loads when B and its rows are 16-byte aligned
-    const bool vb = (reinterpret_cast<uintptr_t>(B) & 15) == 0 && (ldb * int64_t(sizeof(X))) % 16 == 0;
+    // 16-byte B staging loads and vector C rows when B, C and their rows are
+    // 16-byte aligned
+    const bool vb = (reinterpret_cast<uintptr_t>(B) & 15) == 0 && (ldb * int64_t(sizeof(X))) % 16 == 0 &&
+                    (reinterpret_cast<uintptr_t>(C) & 15) == 0 && (ldc * int64_t(sizeof(X))) % 16 == 0;
     auto launch = [&](auto vtag) {
       constexpr bool VB = decltype(vtag)::value;
       if (beta != 0.0)
