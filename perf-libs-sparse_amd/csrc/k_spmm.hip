@@ -558,12 +558,25 @@ __global__ __launch_bounds__(kBlock) void k_bsr_spmm(
 #pragma unroll
   for (int i = 0; i < BS; ++i) {
     X *c = Cm + (int64_t(br) * BS + i) * ldc + c0;
+    if constexpr (VEC) {  // the CPL columns as one vector (C aligned like B)
+      typedef X XC __attribute__((ext_vector_type(CPL)));
+      XC o;
+      if (BETA) o = *reinterpret_cast<const XC *>(c);
 #pragma unroll
-    for (int t = 0; t < CPL; ++t) {
-      if (c0 + t < k) {
+      for (int t = 0; t < CPL; ++t) {
         A r = alpha * acc[i][t];
-        if (BETA) r += beta * A(c[t]);
-        c[t] = X(r);
+        if (BETA) r += beta * A(o[t]);
+        o[t] = X(r);
+      }
+      *reinterpret_cast<XC *>(c) = o;
+    } else {
+#pragma unroll
+      for (int t = 0; t < CPL; ++t) {
+        if (c0 + t < k) {
+          A r = alpha * acc[i][t];
+          if (BETA) r += beta * A(c[t]);
+          c[t] = X(r);
+        }
       }
     }
   }
@@ -790,9 +803,11 @@ void launch_bsr_spmm(const pls::DevBsr &A, int32_t k, double alpha, const void *
   const int64_t waves = (units + RPW - 1) / RPW;
   const unsigned grid = unsigned((waves + kBlock / 64 - 1) / (kBlock / 64));
   if (grid == 0) return;
-  // vector loads of CPL columns: every piece inside a row and aligned
-  const bool vec = CPL > 1 && k % WCOLS == 0 && ldb % CPL == 0 &&
-                   reinterpret_cast<uintptr_t>(B) % (CPL * sizeof(X)) == 0;
+  // vector loads and stores of CPL columns: every piece of B and C inside a
+  // row and aligned
+  const bool vec = CPL > 1 && k % WCOLS == 0 && ldb % CPL == 0 && ldc % CPL == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % (CPL * sizeof(X)) == 0 &&
+                   reinterpret_cast<uintptr_t>(C) % (CPL * sizeof(X)) == 0;
   const bool cm = A.layout == PERFLIBS_COL_MAJOR;
   auto go = [&](auto kern) {
     kern<<<grid, kBlock, 0, s>>>(A.mb, npass, k, A.rp.as<int32_t>(), A.ci.as<int32_t>(),
