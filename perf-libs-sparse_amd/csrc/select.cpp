@@ -7,11 +7,12 @@
 //                (the reference for the other modes' regret);
 //   LEARNED    — a 12 -> 64 -> 64 -> K MLP on matrix features predicts
 //                log(time / best time) per variant; the best three
-//                predictions and CSR-adaptive (the best single variant over
-//                the held-out matrices) are timed like the exhaustive mode and
-//                the fastest wins (the reference's timed audition as the
-//                safety net), and features outside the training range fall
-//                back to the exhaustive sweep;
+//                predictions and the two CSR kernels (CSR-adaptive, the best
+//                single variant over the held-out matrices, and CSR-vector,
+//                the best on small ones; both run on the same device CSR) are
+//                timed like the exhaustive mode and the fastest wins (the
+//                reference's timed audition as the safety net), and features
+//                outside the training range fall back to the exhaustive sweep;
 //   NETWORK    — the MLP's argmin alone, no timing (the ablation of the
 //                audition);
 //   HEURISTIC  — fixed rules on the same features (the no-network baseline);
@@ -360,15 +361,16 @@ static int learned_variant(perflibs_spmat_top_t *A, const Operand &o, const floa
   if (nc == 0) return PERFLIBS_AMD_CSR_VECTOR;
   A->select_path = PERFLIBS_AMD_PATH_NETWORK;
   if (!audition || nc == 1) return order[0];
-  // time the best predictions, and CSR-adaptive (a floor the network's
-  // ranking cannot lose), with the exhaustive mode's protocol
+  // time the best predictions, and the two CSR kernels (a floor the
+  // network's ranking cannot lose), with the exhaustive mode's protocol
   bool only[kK] = {false};
   int na = std::min(nc, kAudition);
   for (int i = 0; i < na; ++i) only[order[i]] = true;
-  if (!only[PERFLIBS_AMD_CSR_ADAPTIVE] && variant_candidate(A, o, PERFLIBS_AMD_CSR_ADAPTIVE)) {
-    only[PERFLIBS_AMD_CSR_ADAPTIVE] = true;
-    ++na;
-  }
+  for (const int v : {PERFLIBS_AMD_CSR_ADAPTIVE, PERFLIBS_AMD_CSR_VECTOR})
+    if (!only[v] && variant_candidate(A, o, v)) {
+      only[v] = true;
+      ++na;
+    }
   double ms[kK];
   if (time_variants(A, o, A->spmv_op, 3, 10, ms, only) != PERFLIBS_STATUS_SUCCESS) {
     (void)hipGetLastError();

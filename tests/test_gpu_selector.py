@@ -144,8 +144,8 @@ def test_time_variants_and_exhaustive_mode(pls, orc, gen):
 def test_learned_mode_uses_model(pls, gen):
     """A model whose output biases rank variant 4, then 7, then 2 first: the
     network mode takes 4; the learned mode times exactly those three and
-    CSR-adaptive and keeps the fastest (the pick is one of them); features
-    outside the stored training range run the exhaustive sweep."""
+    the two CSR kernels and keeps the fastest (the pick is one of them);
+    features outside the stored training range run the exhaustive sweep."""
     rng = np.random.default_rng(3)
     P = (rng.standard_normal(mlp.num_params(K)) * 0.1).astype(np.float32)
     P[: -K] *= 1e-3  # hidden layers near zero: the output biases decide
@@ -168,7 +168,7 @@ def test_learned_mode_uses_model(pls, gen):
         B = pls.SpMat.csr(10000, 10000, rp, ci, v)
         B.select_mode(pls.SELECT_LEARNED)
         B.optimize()
-        assert B.select_info() == (pls.PATH_AUDITION, 4) and B.variant() in (4, 7, 2, 1)
+        assert B.select_info() == (pls.PATH_AUDITION, 5) and B.variant() in (4, 7, 2, 1, 0)
         # a matrix whose features lie outside the range: every candidate timed
         # (on power-law rows the padding rule leaves no SELL-C-sigma candidate)
         rp2, ci2, v2 = gen.powerlaw(1 << 15, 5)
@@ -210,13 +210,13 @@ def test_shipped_model_picks_like_numpy(pls, orc, gen, tmp_path):
         want = int(np.argmin(np.where(cand, pred[0], np.inf)))
         assert A.variant() == want, (kind, pls.VARIANTS[A.variant()], pls.VARIANTS[want])
         # the default mode (a model is loaded) is the learned audition of the
-        # best three and CSR-adaptive
+        # best three and the two CSR kernels
         D = pls.SpMat.csr(mm, n, rp, ci, v)
         D.hint(pls.HINT_SPMV_INVOCATIONS, pls.INVOCATIONS_MANY)
         D.optimize()
         path, ntimed = D.select_info()
         timed = set(np.argsort(np.where(cand, pred[0], np.inf), kind="stable")[:min(3, int(cand.sum()))].tolist())
-        timed.add(1)
+        timed |= {0, 1}
         assert (path == pls.PATH_AUDITION and ntimed == len(timed)
                 and D.variant() in timed) or path == pls.PATH_EXHAUSTIVE
 
@@ -275,7 +275,7 @@ def test_learned_selector_regret_on_regenerated_heldout(pls):
     """Regret on this GPU, not on stored timings: the 50 smallest matrices of the
     held-out set (every generator represented) are regenerated from their seeds,
     every candidate variant is timed here (the exhaustive judge), and the default
-    optimize() (the MLP ranks, its three best and CSR-adaptive are timed) is scored against it:
+    optimize() (the MLP ranks, its three best and the two CSR kernels are timed) is scored against it:
     regret = geomean(t(pick) / t(best)) - 1 <= 10%, below the hand heuristic and
     below every fixed variant that is a candidate on all of them."""
     import time
