@@ -406,8 +406,17 @@ perflibs_status_t perflibs_amd_batch_create(perflibs_amd_batch_t *out, int64_t n
   B->h_rp.resize(size_t(rows_total + nprob));
   const size_t vsz = B->vs == VStore::F64 ? 8 : B->vs == VStore::F32 ? 4 : 2;
   e = B->rp.alloc(sizeof(int32_t) * std::max<size_t>(B->h_rp.size(), 1));
-  if (e == hipSuccess) e = B->ci.alloc(sizeof(int32_t) * size_t(std::max<int64_t>(nnz_total, 1)));
-  if (e == hipSuccess) e = B->vals.alloc(vsz * size_t(std::max<int64_t>(nnz_total, 1)));
+  // one padding entry (column 0, value 0) after the last: the tiles stage
+  // their entries in aligned pairs
+  if (e == hipSuccess) e = B->ci.alloc(sizeof(int32_t) * size_t(nnz_total + 1));
+  if (e == hipSuccess) e = B->vals.alloc(vsz * size_t(nnz_total + 1));
+  if (e == hipSuccess) {
+    const std::vector<uint8_t> z(8, 0);
+    e = hipMemcpy(B->ci.as<int32_t>() + nnz_total, z.data(), sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(static_cast<uint8_t *>(B->vals.get()) + vsz * size_t(nnz_total), z.data(), vsz,
+                    hipMemcpyHostToDevice);
+  }
   DevBuf tmp;  // FP32 staging of one problem's values for the BF16 conversion
   for (int64_t p = 0; p < nprob && e == hipSuccess; ++p) {
     const Operand &o = ops[size_t(p)];

@@ -56,21 +56,34 @@ __device__ __forceinline__ void batch_tile(const BatchTile &tl, const BatchProbl
   const bool staged = nnz <= kBatchStage && nrows <= kBatchRows<A>;
   __syncthreads();  // the previous tile is done with the LDS
   if (staged) {
-    constexpr int PER = kBatchStage / kBlock;
-    int32_t c[PER];
-    A a[PER];
+    if (nnz > 0) {
+      // entry pairs (one 8-byte index load, one 2-value load) from the even
+      // absolute index at or below the tile's first entry; the batch arrays
+      // end with one padding entry, so the last pair stays in bounds
+      constexpr int NP = kBatchStage / kBlock / 2;
+      const int64_t g0 = int64_t(pb.nz_off) + e0;
+      const int32_t sh = int32_t(g0 & 1);
+      const int32_t *cia = ci_all + (g0 - sh);
+      const V *vaa = val_all + (g0 - sh);
+      const int32_t lastp = (nnz + sh - 1) & ~1;  // last pair holding a tile entry
+      int32_t c[NP][2];
+      A a[NP][2];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int32_t q = min(tid + u * kBlock, max(nnz - 1, 0));
-      c[u] = nnz > 0 ? ldi(ci, e0 + q) : 0;
-      a[u] = nnz > 0 ? alpha * A(ldv(va, e0 + q)) : A(0);
-    }
+      for (int u = 0; u < NP; ++u) ld_pair(cia, vaa, min(2 * (tid + u * kBlock), lastp), c[u], a[u]);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int32_t q = tid + u * kBlock;
-      if (q < nnz) {
-        lds.ci[q] = c[u];
-        lds.a[q] = a[u];
+      for (int u = 0; u < NP; ++u) {
+        const int32_t q = 2 * (tid + u * kBlock) - sh;  // tile position of the pair's first entry
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (q + h >= 0 && q + h < nnz) {
+            lds.ci[q + h] = c[u][h];
+            lds.a[q + h] = alpha * a[u][h];
+          }
+      }
+      // a full tile from an odd index has one entry past the pairs
+      if (sh && tid == 0 && nnz == kBatchStage) {
+        lds.ci[nnz - 1] = ldi(ci, e0 + nnz - 1);
+        lds.a[nnz - 1] = alpha * A(ldv(va, e0 + nnz - 1));
       }
     }
     for (int32_t q = tid; q <= nrows; q += kBlock) lds.rp[q] = rp[tl.r0 + q] - e0;
