@@ -82,6 +82,20 @@ __device__ __forceinline__ void gather_products(const int32_t *__restrict__ ci,
   for (int k = 0; k < N; ++k) prod[k] = a[k] * acc_t<X>(x[c[k]]);
 }
 
+// Values j and j + 1 (j even) as one 2 * sizeof(V)-byte non-temporal load.
+template <class V, class A>
+__device__ __forceinline__ void ld_val_pair(const V *__restrict__ val, int32_t j, A (&a)[2]) {
+  typedef V V2 __attribute__((ext_vector_type(2)));
+  const V2 vv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(val + j));
+  if constexpr (sizeof(V) == 2) {  // BF16: the upper half of an FP32 word
+    a[0] = A(__uint_as_float(uint32_t(vv.x) << 16));
+    a[1] = A(__uint_as_float(uint32_t(vv.y) << 16));
+  } else {
+    a[0] = A(vv.x);
+    a[1] = A(vv.y);
+  }
+}
+
 // Entries j and j + 1 (j even): column indices as one 8-byte load, values as
 // one 2 * sizeof(V)-byte load, both non-temporal.
 template <class V, class A>

@@ -239,9 +239,26 @@ __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     return;
   }
 
-  for (int32_t q = tid; q < nnz; q += kBlock) {
-    s_a[q] = alpha * A(ldv(bval, e0 + q));
-    s_l[q] = lidx[e0 + q];
+  {
+    // values and list positions in aligned pairs from the even index at or
+    // below e0 (value and position arrays end with one padding entry)
+    const int32_t a0 = e0 & ~1, sh = e0 - a0;
+    typedef uint8_t L2 __attribute__((ext_vector_type(2)));
+    for (int32_t t = tid; 2 * t < nnz + sh; t += kBlock) {
+      A v[2];
+      const int32_t j = a0 + 2 * t;
+      ld_val_pair(bval, j, v);
+      const L2 l = *reinterpret_cast<const L2 *>(lidx + j);
+      const int32_t q = 2 * t - sh;
+      if (q >= 0) {
+        s_a[q] = alpha * v[0];
+        s_l[q] = l.x;
+      }
+      if (q + 1 < nnz) {
+        s_a[q + 1] = alpha * v[1];
+        s_l[q + 1] = l.y;
+      }
+    }
   }
   for (int32_t q = tid; q <= nrows; q += kBlock) {
     s_rp[q] = brp[r0 + q] - e0;

@@ -363,13 +363,17 @@ hipError_t upload_blocks(const Operand &o, const LbHost &h, int32_t umax, DevCsr
   if (e == hipSuccess) e = d.lb_uoff.upload(h.uoff.data(), sizeof(int32_t) * h.uoff.size());
   if (e == hipSuccess)
     e = d.lb_ucol.upload(h.ucol.data(), sizeof(int32_t) * std::max<size_t>(h.ucol.size(), 1));
+  // list positions and block-order values end with one padding entry (the
+  // kernel stages them in aligned pairs); the CSR plan's values carry theirs
   if (h.brow.empty()) {
-    if (e == hipSuccess) e = d.lb_lidx.upload(h.lidx.data(), h.lidx.size());
+    std::vector<uint8_t> lidx(h.lidx);
+    lidx.push_back(0);
+    if (e == hipSuccess) e = d.lb_lidx.upload(lidx.data(), lidx.size());
   } else {
     const int32_t m = o.m, base = o.prp[0];
     const size_t nnz = h.lidx.size();
-    std::vector<int32_t> prp(size_t(m) + 1, 0), epos(nnz, 0);
-    std::vector<uint8_t> lidx(nnz, 0);
+    std::vector<int32_t> prp(size_t(m) + 1, 0), epos(nnz + 1, 0);
+    std::vector<uint8_t> lidx(nnz + 1, 0);
     for (int32_t i = 0; i < m; ++i) {
       const int32_t r = h.brow[size_t(i)], j0 = o.prp[r] - base, len = o.prp[r + 1] - o.prp[r];
       prp[size_t(i) + 1] = prp[size_t(i)] + len;
@@ -382,11 +386,13 @@ hipError_t upload_blocks(const Operand &o, const LbHost &h, int32_t umax, DevCsr
     if (e == hipSuccess) e = d.lb_lidx.upload(lidx.data(), lidx.size());
     if (e == hipSuccess) e = d.lb_brow.upload(h.brow.data(), sizeof(int32_t) * size_t(m));
     if (e == hipSuccess) e = d.lb_prp.upload(prp.data(), sizeof(int32_t) * prp.size());
-    if (e == hipSuccess) e = d.lb_epos.upload(epos.data(), sizeof(int32_t) * nnz);
-    if (e == hipSuccess) e = d.lb_val.alloc(es * nnz);
+    // the padding entry gathers the CSR plan's own padding value (zero)
+    epos[nnz] = int32_t(nnz);
+    if (e == hipSuccess) e = d.lb_epos.upload(epos.data(), sizeof(int32_t) * (nnz + 1));
+    if (e == hipSuccess) e = d.lb_val.alloc(es * (nnz + 1));
     if (e == hipSuccess)
       e = plk::gather_vals(d.vs, d.lb_val.get(), d.vals.get(), d.lb_epos.as<int32_t>(),
-                           int64_t(nnz), stream());
+                           int64_t(nnz + 1), stream());
   }
   d.lb_umax = umax;
   d.lb_state = e == hipSuccess ? 1 : 0;
@@ -467,7 +473,7 @@ int spmm_blocks_host(const Operand &o, int rows, int32_t *nblocks, int32_t *list
 
 hipError_t refresh_block_values(DevCsr &d) {
   if (d.lb_state != 1 || !d.lb_clustered || !d.lb_val.get()) return hipSuccess;
-  return plk::gather_vals(d.vs, d.lb_val.get(), d.vals.get(), d.lb_epos.as<int32_t>(), d.nnz,
+  return plk::gather_vals(d.vs, d.lb_val.get(), d.vals.get(), d.lb_epos.as<int32_t>(), d.nnz + 1,
                           stream());
 }
 
