@@ -413,15 +413,25 @@ namespace {
 bool choose_blocks(const Operand &o, int rows, LbHost &out, int32_t &umax_out) {
   const int64_t nnz = int64_t(o.prp[o.m]) - o.prp[0];
   std::vector<int32_t> cp, cr;
+  Overlap ov;
+  // the column -> rows structure and the sampled overlap are built on a thread
+  // of their own while the consecutive cut runs
+  const bool may_cluster = rows != 1 && o.m >= 2;
+  std::thread pre;
+  if (may_cluster)
+    pre = std::thread([&] {
+      rows_by_column(o, cp, cr);
+      ov = sampled_overlap(o, cp, cr);
+    });
   int overlap = -1;     // 1: rows share enough columns for clustering to pay
   int64_t ranges = 16;  // row ranges clustered in parallel
+  bool used = false;
   for (const int32_t umax : {kLbUcolsSmall, kLbUcolsLarge}) {
     LbHost h;
     cut_consecutive(o, umax, h);
-    if (rows != 1 && double(h.direct) < 0.4 * double(nnz) && o.m >= 2) {
+    if (may_cluster && double(h.direct) < 0.4 * double(nnz)) {
       if (overlap < 0) {
-        rows_by_column(o, cp, cr);
-        const Overlap ov = sampled_overlap(o, cp, cr);
+        pre.join();
         overlap = rows == 2 || ov.shared >= 0.25 ? 1 : 0;
         // rows whose partners are far apart (a scattered row order) are
         // clustered as one range when that stays quick
@@ -432,12 +442,13 @@ bool choose_blocks(const Operand &o, int rows, LbHost &out, int32_t &umax_out) {
           (rows == 2 || double(c.cost()) < 0.9 * double(h.cost())))
         h = std::move(c);
     }
-    const bool used = double(h.cost()) <= 0.6 * double(nnz);
+    used = double(h.cost()) <= 0.6 * double(nnz);
     out = std::move(h);
     umax_out = umax;
-    if (used) return true;
+    if (used) break;
   }
-  return false;
+  if (pre.joinable()) pre.join();
+  return used;
 }
 }  // namespace
 
