@@ -178,16 +178,18 @@ __global__ __launch_bounds__(kBlock) void k_spmm_blk(
 // rows, consecutive ones or (brow != nullptr) the rows brow lists for the
 // block, whose non-zeros the plan keeps in block order (prp, lval, lidx: one
 // contiguous range per block, like consecutive rows).  The block's
-// alpha-scaled values and one-byte list
-// positions are staged in LDS once; then, KC columns at a time, every B row the
-// block needs is loaded into LDS once (coalesced, KC contiguous elements per
-// row) and each row of C is accumulated from LDS by a group of KC / CPL lanes
-// (CPL adjacent columns each), 8 non-zeros in flight per lane.  On a stencil
-// a B row feeds about 2.7 non-zeros of the block, so L2 -> CU traffic for B
-// falls by that factor against the direct kernels, which fetch a B row per
-// non-zero.  A block without a list (one row with more than UMAX distinct
-// columns or lb_nnz_per_ucol(UMAX) * UMAX non-zeros) spreads its non-zeros over the workgroup's
-// lane groups, reads B directly and sums the groups through LDS.
+// alpha-scaled values and one-byte list positions are staged in LDS once (in
+// aligned pairs); then, KC columns at a time, every B row the block needs is
+// loaded into LDS once (coalesced, KC contiguous elements per row, 16 bytes per
+// lane when VB) and each row of C is accumulated from LDS by a group of
+// KC / CPL lanes (CPL adjacent columns each, written as one vector when VB),
+// 8 non-zeros in flight per lane.  On the 27-point stencil a staged B row
+// feeds about 5 non-zeros of a block of clustered rows (2.6 with consecutive
+// rows), so L2 -> CU traffic for B falls by that factor against the direct
+// kernels, which fetch a B row per non-zero.  A block without a list (one row
+// with more than UMAX distinct columns or lb_nnz_per_ucol(UMAX) * UMAX
+// non-zeros) spreads its non-zeros over the workgroup's lane groups, reads B
+// directly and sums the groups through LDS.
 template <int KC, int UMAX, int CPL, class V, class X, bool BETA, bool VB>
 __global__ __launch_bounds__(kBlock) void k_spmm_lds(
     const int32_t *__restrict__ row0s, const int32_t *__restrict__ uoff,
