@@ -375,13 +375,19 @@ hipError_t upload_blocks(const Operand &o, const LbHost &h, int32_t umax, DevCsr
     std::vector<int32_t> prp(size_t(m) + 1, 0), epos(nnz + 1, 0);
     std::vector<uint8_t> lidx(nnz + 1, 0);
     for (int32_t i = 0; i < m; ++i) {
-      const int32_t r = h.brow[size_t(i)], j0 = o.prp[r] - base, len = o.prp[r + 1] - o.prp[r];
-      prp[size_t(i) + 1] = prp[size_t(i)] + len;
-      for (int32_t t = 0; t < len; ++t) {
-        epos[size_t(prp[size_t(i)] + t)] = j0 + t;
-        lidx[size_t(prp[size_t(i)] + t)] = h.lidx[size_t(j0 + t)];
-      }
+      const int32_t r = h.brow[size_t(i)];
+      prp[size_t(i) + 1] = prp[size_t(i)] + (o.prp[r + 1] - o.prp[r]);
     }
+    parallel_ranges(m, 65536, 16, [&](int, int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) {
+        const int32_t r = h.brow[size_t(i)], j0 = o.prp[r] - base;
+        const int32_t q0 = prp[size_t(i)], len = prp[size_t(i) + 1] - q0;
+        for (int32_t t = 0; t < len; ++t) {
+          epos[size_t(q0 + t)] = j0 + t;
+          lidx[size_t(q0 + t)] = h.lidx[size_t(j0 + t)];
+        }
+      }
+    });
     const size_t es = d.vs == VStore::F64 ? 8 : d.vs == VStore::F32 ? 4 : 2;
     if (e == hipSuccess) e = d.lb_lidx.upload(lidx.data(), lidx.size());
     if (e == hipSuccess) e = d.lb_brow.upload(h.brow.data(), sizeof(int32_t) * size_t(m));
