@@ -133,15 +133,17 @@ struct DevCsr {
 // Limits of a column-compressed SpMM block (must match k_spmm.hip): distinct
 // columns per list (128, or 256 when lists of 128 leave too many rows without
 // a list; positions fit a byte), non-zeros and rows.  Non-zeros per list
-// column: 6 for lists of 128 (27-point stencil, clustered rows, FP64 k = 8 /
-// 32 / 128: 0.39 / 0.85 / 2.81 ms with 4, 0.33 / 0.79 / 2.59 with 6, 0.31 /
-// 0.83 / 2.65 with 8: a larger block shares more of its columns, a smaller one
-// leaves the LDS room for more workgroups per CU), 8 for lists of 256, so a
-// whole 8 x 8 block row (8 rows of 216 non-zeros under 216 columns) is one
-// block (8 x 8-block operator, k = 32: 6.5 -> 5.0 ms).
+// column: 7 for lists of 128 (27-point stencil, clustered rows, FP64 k = 8 /
+// 32 / 128 with one-element staging: 0.39 / 0.85 / 2.81 ms with 4, 0.33 / 0.79
+// / 2.59 with 6, 0.31 / 0.83 / 2.65 with 8; with 16-byte staging 7 measured
+// 0.29 / 0.71 / 2.29 against 0.30 / 0.71 / 2.34 with 6: a larger block shares
+// more of its columns, a smaller one leaves the LDS room for more workgroups
+// per CU), 8 for lists of 256, so a whole 8 x 8 block row (8 rows of 216
+// non-zeros under 216 columns) is one block (8 x 8-block operator, k = 32:
+// 6.5 -> 5.0 ms).
 constexpr int32_t kLbUcolsSmall = 128;
 constexpr int32_t kLbUcolsLarge = 256;
-constexpr int32_t lb_nnz_per_ucol(int32_t umax) { return umax > kLbUcolsSmall ? 8 : 6; }
+constexpr int32_t lb_nnz_per_ucol(int32_t umax) { return umax > kLbUcolsSmall ? 8 : 7; }
 constexpr int32_t kLbRows = 128;
 
 // Device SELL-C-sigma (layout of src/sell_c_sigma.hpp:43-104, NOTRANS).

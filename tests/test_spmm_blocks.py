@@ -1,6 +1,6 @@
 """Row blocks of the wide-B SpMM plan, inspected on the host (no device):
 consecutive and clustered cuts of CSR patterns checked against their limits
-(every row in exactly one block, at most list_max distinct columns and 6 *
+(every row in exactly one block, at most list_max distinct columns and 7 *
 list_max non-zeros per list block of lists of 128, 8 * list_max for lists of
 256, at most 128 rows), list totals recounted from the blocks, and the choice
 between the cuts (clustering on stencils, in natural and scrambled row order;
@@ -18,7 +18,7 @@ def blocks_ok(rp, ci, info):
     assert np.array_equal(np.sort(order), np.arange(m))
     assert starts[0] == 0 and starts[-1] == m and np.all(np.diff(starts) >= 1)
     umax = info["list_max"]
-    nnz_max = (6 if umax == 128 else 8) * umax
+    nnz_max = (7 if umax == 128 else 8) * umax
     base = rp[0]
     total = 0
     for b in range(info["nblocks"]):
