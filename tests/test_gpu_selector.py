@@ -274,7 +274,7 @@ def test_block_fill_features_sampled_block_rows(pls, gen):
 def test_learned_selector_regret_on_regenerated_heldout(pls):
     """Regret on this GPU, not on stored timings: the 50 smallest matrices of the
     held-out set (every generator represented) are regenerated from their seeds,
-    every candidate variant is timed here (the exhaustive judge), and the default
+    every candidate variant is timed here (the exhaustive sweep), and the default
     optimize() (the MLP ranks, its three best and the two CSR kernels are timed) is scored against it:
     regret = geomean(t(pick) / t(best)) - 1 <= 10%, below the hand heuristic and
     below every fixed variant that is a candidate on all of them."""
