@@ -255,6 +255,16 @@ def test_introspection_without_plans(pls):
     assert lib.perflibs_amd_spmm_info(A.h, *[C.byref(x) for x in v[:3]], C.byref(tot)) == 1
     assert lib.perflibs_amd_panel_info(A.h, *[C.byref(x) for x in v]) == 1
     assert lib.perflibs_amd_spmm_info(None, None, None, None, None) == 1
+    clustered = C.c_int(7)
+    assert lib.perflibs_amd_spmm_rows_clustered(A.h, C.byref(clustered)) == 1
+    assert clustered.value == 7  # left alone
+    assert lib.perflibs_amd_spmm_rows_clustered(None, C.byref(clustered)) == 1
+    # the row-block mode is checked and stored without a device plan
+    for rows in (0, 1, 2):
+        assert lib.perflibs_amd_set_spmm_rows(A.h, rows) == 0
+    for rows in (-1, 3):
+        assert lib.perflibs_amd_set_spmm_rows(A.h, rows) == 1
+    assert lib.perflibs_amd_set_spmm_rows(None, 0) == 1
 
 
 def test_noalloc_backward_refuses_sizes_beyond_its_scratch(pls):
